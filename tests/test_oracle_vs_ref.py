@@ -179,11 +179,12 @@ def test_push_and_reorder(ref, plasma):
     part = plasma.copy()
     qpo.extpsi(part, smooth_fields(np.random.default_rng(10), 1, 0.3), -1.0, DEX)
     ppart, kpic, _ = ref.to_tiles(part.copy(), NX, NY)
-    err = qpo.push(part, fxy, bxy, psit, -1.0, DXI, DEX)
+    dt = 0.9  # large step: O(1 cell) moves, across tile edges and the walls
+    err = qpo.push(part, fxy, bxy, psit, -1.0, dt, DEX)
     kp = qpo.order_by_tile(NX, NY, part)
     ppart, kpic, irc = ref.push_and_reorder(
         ppart, kpic, ref.r_from_oracle(fxy, NX, NY), ref.r_from_oracle(bxy, NX, NY),
-        ref.r_from_oracle(psit, NX, NY), -1.0, DXI, DEX, NX, NY)
+        ref.r_from_oracle(psit, NX, NY), -1.0, dt, DEX, NX, NY)
     assert err == 0 and irc == 0
     assert np.array_equal(kp, kpic)               # per-tile counts, bit-exact
     rp = ref.from_tiles(ppart, kpic)
@@ -196,7 +197,8 @@ def test_push_and_reorder(ref, plasma):
     # the push really moved particles between tiles and across the walls
     kp0 = np.bincount(qpo.tile_of(NX, NY, plasma) - 1, minlength=len(kp))
     assert not np.array_equal(kp0, kp)
-    assert (rp[:, 0] < 1.0).any() or (rp[:, 0] >= NX - 1).any()
+    assert (rp[:, 0] < 1.0).any() and (rp[:, 0] >= NX - 1).any()
+    assert (rp[:, 1] < 1.0).any()
 
 
 def test_ranorm_bit_exact():
