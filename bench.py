@@ -1,0 +1,220 @@
+"""xi-slice throughput of the quasi-static solver on one MI355X.
+
+Workload (the precision-form configuration): 1024 x 1024 transverse grid over
+x, y in [-10, 10] c/wp, xi in [0, 10] c/wp with 1024 slices, uniform plasma with
+4 x 4 particles per cell, a bi-Gaussian electron driver of 2^25 particles
+(nb/n0 = 10, sigma_r = 0.3, sigma_z = 0.5, gamma = 2e4, centred at xi = 2).
+A step is one xi slice: beam-charge copy, charge deposit, psi, E_perp, the
+B_perp predictor-corrector to relative change 1e-6 (at most 20 passes), the
+plasma push with its cell sort, and the slice writes into the 3-D wake.  The
+slices timed lie in the beam (the xi range where the corrector works); the
+slices before it are run untimed.  Inputs are resident on the GPU when timing
+starts; the clock is bracketed by stream synchronisations.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "quickpic-opensource_amd")
+for _p in (REPO, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = ("ξ-slices/s at 1024², 16 ppc; CNN-seeded FP32 vs plain FP64 at same B "
+          "residual")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kind, qps, np_, n, esize, launches_per_slice=None):
+    """Bytes one launch of a kernel class must move (semantic operands, read or
+    written once): particle words times the particle count plus field planes
+    times the node count."""
+    nodes = (n + 1) * (n + 1)
+    table = {
+        qps.PROF_AMJDP: 8 * np_ + 14 * nodes,      # x,y,px,py,p6,p7,q read, p6 written
+        qps.PROF_PUSH: 12 * np_ + 6 * nodes,       # 7 read, 5 written; 6 field planes
+        qps.PROF_DEPOSIT_Q: 3 * np_ + nodes,
+        qps.PROF_EXTPSI: 6 * np_ + nodes,
+        qps.PROF_SORT: 16 * np_,                   # 8 words read and written
+    }
+    if kind not in table:
+        return None
+    return table[kind] * esize
+
+
+def run_gpu(args, precision, beam_part=None):
+    import numpy as np
+
+    import qpslice as qps
+    ind = args.ind
+    ctx = qps.Ctx(ind, ind, args.indz, precision=precision)
+    n = ctx.nx
+    t0 = time.time()
+    sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
+    bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0), sigma=(0.3, 0.3, 0.5), sigma_v=(1.0, 1.0, 0.0),
+                  np_=tuple(args.beam_np), gamma=2.0e4, peak=args.peak, dt=10.0,
+                  npmax=0 if beam_part is None else beam_part.shape[0] + 2)
+    if beam_part is not None:
+        bm.upload(beam_part)
+    sim = qps.Sim(ctx, [sp], [bm], iter_=2)
+    log(f"[bench] {'FP32' if precision == qps.FP32 else 'FP64'} setup {time.time() - t0:.1f}s "
+        f"(plasma {sp.np}, beam {bm.np})")
+    sim.begin_step()
+    # first timed slice: a little ahead of the beam centre (xi = 2)
+    j_first = max(1, int(round((2.0 - 2.0 * 0.5) / ctx.dxi)))
+    t0 = time.time()
+    for j in range(1, j_first - args.warmup):
+        sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+    log(f"[bench] fast-forward to slice {j_first - args.warmup}: {time.time() - t0:.1f}s")
+    for j in range(max(1, j_first - args.warmup), j_first):
+        sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+    ctx.sync()
+    ctx.profile(True)
+    passes = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for j in range(j_first, j_first + args.steps):
+        it, _ = sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+        passes.append(it)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    prof = {qps.PROF_NAMES[k]: ctx.profile_get(k) for k in range(len(qps.PROF_NAMES))}
+    ctx.profile(False)
+    esize = 4 if precision == qps.FP32 else 8
+    # dominant kernel class by accumulated time
+    dom = max(range(len(qps.PROF_NAMES)), key=lambda k: prof[qps.PROF_NAMES[k]][0])
+    ms, cnt = prof[qps.PROF_NAMES[dom]]
+    alg = algorithmic_bytes(dom, qps, sp.np, n, esize)
+    roof = None
+    if alg is not None and cnt > 0:
+        avg_s = ms / cnt / 1e3
+        ach = alg / avg_s / 1e9
+        roof = {"kernel": qps.PROF_NAMES[dom], "bound": "hbm", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_ms": round(ms / cnt, 4),
+                "algorithmic_bytes_per_launch": alg}
+    # all kernel classes, for the record
+    classes = {}
+    for k, name in enumerate(qps.PROF_NAMES):
+        t, c = prof[name]
+        if c:
+            entry = {"ms_total": round(t, 3), "launches": c, "ms_avg": round(t / c, 4)}
+            b = algorithmic_bytes(k, qps, sp.np, n, esize)
+            if b:
+                entry["GBs"] = round(b / (t / c / 1e3) / 1e9, 1)
+            classes[name] = entry
+    result = {
+        "slices_per_s": args.steps / elapsed,
+        "ms_per_slice": elapsed / args.steps * 1e3,
+        "passes_mean": float(np.mean(passes)),
+        "passes": passes,
+        "first_slice": j_first,
+        "roofline": roof,
+        "kernels": classes,
+        "np_plasma": sp.np,
+        "np_beam": bm.np,
+    }
+    part = bm.download() if beam_part is None and args.fp64 else None
+    ctx.close()
+    return result, part
+
+
+def cpu_baseline(args):
+    """The oracle (FP64, one thread) on a bounded sample of the same workload:
+    two xi slices at the full transverse size and ppc, 2 corrector passes,
+    no beam (the per-slice CPU cost does not depend on the beam)."""
+    from oracle import qpo
+    t0 = time.time()
+    grid = qpo.Grid(args.ind, args.ind, 4)
+    sim = qpo.Simulation(grid, [qpo.SpeciesSpec(ppc=(args.ppc, args.ppc))], [], dt=10.0,
+                         iter_=2)
+    sim.begin_step()
+    log(f"[bench] cpu oracle setup {time.time() - t0:.1f}s")
+    nsl = args.cpu_slices
+    t0 = time.perf_counter()
+    for j in range(1, nsl + 1):
+        sim.slice(j)
+    el = time.perf_counter() - t0
+    return {"value": round(nsl / el, 4), "unit": "xi-slices/s", "cores": 1, "kind": "port",
+            "sample": f"{nsl} xi slices of the FP64 C oracle at {grid.nx}x{grid.ny}, "
+                      f"{args.ppc}x{args.ppc} ppc, 2 corrector passes, one thread "
+                      f"({el:.1f}s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ind", type=int, default=10, help="log2 of the transverse grid")
+    ap.add_argument("--indz", type=int, default=10)
+    ap.add_argument("--ppc", type=int, default=4, help="particles per cell per dimension")
+    ap.add_argument("--beam-np", type=int, nargs=3, default=[128, 128, 2048])
+    ap.add_argument("--peak", type=float, default=10.0)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--iter-max", type=int, default=20)
+    ap.add_argument("--no-fp64", dest="fp64", action="store_false")
+    ap.add_argument("--no-cpu", dest="cpu", action="store_false")
+    ap.add_argument("--cpu-slices", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    import qpslice as qps
+    r32, beam = run_gpu(args, qps.FP32)
+    log(f"[bench] FP32: {r32['slices_per_s']:.1f} slices/s, passes {r32['passes_mean']:.2f}")
+    r64 = None
+    if args.fp64:
+        r64, _ = run_gpu(args, qps.FP64, beam_part=beam)
+        log(f"[bench] FP64: {r64['slices_per_s']:.1f} slices/s, passes {r64['passes_mean']:.2f}")
+    cpu = cpu_baseline(args) if args.cpu else None
+    out = {
+        "metric": METRIC,
+        "value": round(r32["slices_per_s"], 3),
+        "unit": "xi-slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_slice"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded beam, quiet-start plasma)",
+        "config": {
+            "workload": f"precision form: {2 ** args.ind}^2 x {2 ** args.indz} slices, "
+                        f"{args.ppc * args.ppc} ppc, beam {r32['np_beam']} particles, "
+                        f"nb/n0={args.peak}, B residual {args.tol:g} (max {args.iter_max} "
+                        "passes), plain seed (no CNN)",
+            "grid": [2 ** args.ind, 2 ** args.ind, 2 ** args.indz],
+            "ppc": args.ppc * args.ppc,
+            "plasma_particles": r32["np_plasma"],
+            "beam_particles": r32["np_beam"],
+            "passes_per_slice_mean": r32["passes_mean"],
+            "first_timed_slice": r32["first_slice"],
+        },
+        "roofline": r32["roofline"],
+        "cpu_baseline": cpu,
+        "kernels": r32["kernels"],
+    }
+    if r64 is not None:
+        out["fp64_plain"] = {"value": round(r64["slices_per_s"], 3),
+                             "ms_per_step": round(r64["ms_per_slice"], 4),
+                             "passes_per_slice_mean": r64["passes_mean"],
+                             "fp32_over_fp64": round(r32["slices_per_s"] / r64["slices_per_s"], 3),
+                             "roofline": r64["roofline"]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,253 @@
+/*
+ * qpslice.h -- C ABI of libqpslice.so, the MI355X (gfx950) quasi-static
+ * xi-slice solver.
+ *
+ * The entry points mirror the reference's Fortran-2003 type-bound methods on
+ * the hot path, so a Fortran (ISO_C_BINDING), C, C++ or Python caller can
+ * replace those classes one for one:
+ *   species2d  -> qp_species_*   (species2d_class.f03:37-51)
+ *   field2d    -> qp_field2d_*   (field2d_class.f03:38-62)
+ *   field3d    -> qp_field3d_*   (field3d_class.f03, ufield3d_class.f03:396-490)
+ *   beam3d     -> qp_beam_*      (beam3d_class.f03:35-43)
+ *   the xi-slice loop of go_simulation -> qp_sim_* / qp_slice_run
+ *                                 (simulation_class.f03:611-765)
+ * Conventions kept from the reference: component indices (dim, dim1, dim2,
+ * sdim, ddim) and 3-D slice positions (lpos, pos, slice) are 1-based; field
+ * states are 0 = real space, 1 = k space; kinds of fftrk/fftkr are 1, 2, 3.
+ *
+ * All device memory is owned by the library.  Host data crosses only through
+ * the upload/download calls, always as double, in Fortran column-major order:
+ *   real-space field2d   f(dim, ld1 >= nx+1, ld2 >= ny+1)   (ufield2d rs)
+ *   k-space field2d      g(dim, ld1 >= ny+1, ld2 >= nx+1)   (ufield2d ks)
+ *   field3d              f(dim, ld1 >= nx+1, ld2 >= ny+1, ld3 >= nz+1)
+ *   plasma particles     p(8, np): x, y, px, py, pz, p6, p7, q
+ *   beam particles       p(7, np): x, y, z, px, py, pz, q
+ * so the reference's own arrays (rf(dim,nx+2,2*ny+1), rf(dim,ny+1,nx+1),
+ * rf(dim,nx+2,ny+1,nz+1), part(8,npmax), part(7,npmax)) pass unchanged.
+ *
+ * Every call returns an int status: QP_OK, one of the reference's particle
+ * manager classes irc2(1) = 1..4 (part2d_class.f03:410-579), or a negative
+ * error; qp_last_error() describes the last failure of the calling thread.
+ * One HIP stream per context; a context and its objects are not thread-safe;
+ * distinct contexts are independent.
+ */
+#ifndef QPSLICE_H
+#define QPSLICE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qp_ctx qp_ctx;
+typedef struct qp_field2d qp_field2d;
+typedef struct qp_field3d qp_field3d;
+typedef struct qp_species qp_species;
+typedef struct qp_beam qp_beam;
+typedef struct qp_sim qp_sim;
+
+enum {
+    QP_OK = 0,
+    QP_IRC_IHOLE = 1,   /* hole-list overflow  (part2d_class.f03:414)  */
+    QP_IRC_PPBUFF = 2,  /* tile buffer overflow (part2d_class.f03:433) */
+    QP_IRC_SBUF = 3,    /* send buffer overflow (part2d_class.f03:456) */
+    QP_IRC_PPART = 4,   /* tile capacity overflow (part2d_class.f03:527) */
+    QP_ERR_ARG = -1,
+    QP_ERR_HIP = -2,
+    QP_ERR_STATE = -3,  /* field in the wrong space (field2d_class.f03:167) */
+    QP_ERR_ALLOC = -4,
+    QP_ERR_CHECK = -5,  /* particle outside its tile (PPPCHECK2L) */
+    QP_ERR_UNSUPPORTED = -6
+};
+
+enum { QP_FP64 = 0, QP_FP32 = 1 };
+
+/* ---------------------------------------------------------------------- */
+/* context: the grid descriptors spect2d/spect3d (spect3d_class.f03:16-30) */
+/* and the box of simulation%new (simulation_class.f03:165-193)            */
+/* ---------------------------------------------------------------------- */
+/* box = {xmin, xmax, ymin, ymax, zmin, zmax} in c/wp; grid 2^indx x 2^indy
+ * x 2^indz, conducting boundaries (psolver 1), linear interpolation
+ * (inorder 1).  precision: QP_FP64 (the reference's DBL build) or QP_FP32
+ * (FP32 push/deposit/FFT; FP64 beam positions and reductions). */
+int qp_ctx_create(int indx, int indy, int indz, const double box[6],
+                  int precision, int device, qp_ctx **out);
+int qp_ctx_destroy(qp_ctx *ctx);
+int qp_ctx_sync(qp_ctx *ctx);
+int qp_ctx_info(const qp_ctx *ctx, int *nx, int *ny, int *nz, int *precision,
+                double *dex, double *dxi);
+const char *qp_last_error(void);
+
+/* Kernel timing with HIP events on the context stream, by kernel class.
+ * No counterpart on the reference's hot path (its nearest are the wall-clock
+ * timers dtimer.c:4-28 and PWTIMERA, fft2d_lib77.f:4518-4521).  While enabled
+ * every launch of a class is bracketed by two events; qp_ctx_profile_get
+ * synchronises the stream and returns the accumulated time and launch count. */
+enum { QP_PROF_FFT = 0, QP_PROF_TRANSPOSE, QP_PROF_KSPACE, QP_PROF_ELEMENTWISE,
+       QP_PROF_DEPOSIT_Q, QP_PROF_EXTPSI, QP_PROF_AMJDP, QP_PROF_PUSH,
+       QP_PROF_SORT, QP_PROF_BORDER, QP_PROF_BEAM_DEPOSIT, QP_PROF_BEAM_PUSH,
+       QP_PROF_COUNT };
+int qp_ctx_profile(qp_ctx *ctx, int enable);   /* enabling also resets */
+int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count);
+
+/* ---------------------------------------------------------------------- */
+/* field2d (field2d_class.f03)                                             */
+/* ---------------------------------------------------------------------- */
+/* init_field2d :81-135 with fftflag = .true.; state 0 or 1 */
+int qp_field2d_create(qp_ctx *ctx, int dim, int state, qp_field2d **out);
+int qp_field2d_destroy(qp_field2d *f);
+int qp_field2d_state(const qp_field2d *f, int *state, int *dim);
+/* space: 0 = real-space array, 1 = k-space array; sets the state */
+int qp_field2d_upload(qp_field2d *f, int space, const double *host, int ld1,
+                      int ld2);
+int qp_field2d_download(const qp_field2d *f, int space, double *host, int ld1,
+                        int ld2);
+int qp_field2d_as(qp_field2d *f, double value);                    /* asc :493 */
+int qp_field2d_asa(qp_field2d *f, const qp_field2d *that);         /* asa :624 */
+int qp_field2d_fftrk(qp_field2d *f, int kind);                     /* :157-191 */
+int qp_field2d_fftkr(qp_field2d *f, int kind);                     /* :193-227 */
+int qp_field2d_div(qp_field2d *f, qp_field2d *out);                /* :229-249 */
+int qp_field2d_grad(qp_field2d *f, qp_field2d *out);               /* :251-271 */
+int qp_field2d_pot(qp_field2d *q, qp_field2d *out);                /* :295-316 */
+int qp_field2d_smooth(qp_field2d *q, qp_field2d *out);             /* :341-362 */
+int qp_field2d_elf(qp_field2d *q, qp_field2d *out);                /* :364-385 */
+int qp_field2d_bf(qp_field2d *cu, qp_field2d *out);                /* :387-408 */
+/* bfield_qp :410-432: B_perp update into bxy; dcu is overwritten */
+int qp_field2d_bfqp(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
+                    qp_field2d *bxy, double c, double dex);
+/* sum1/sum2, minus1/minus2 (:665-760): n = 0 means every component;
+ * otherwise this(dim(i)) = a1(dim1(i)) op a2(dim2(i)), i = 1..n */
+int qp_field2d_add(qp_field2d *f, const qp_field2d *a1, const qp_field2d *a2,
+                   int n, const int *dim, const int *dim1, const int *dim2);
+int qp_field2d_sub(qp_field2d *f, const qp_field2d *a1, const qp_field2d *a2,
+                   int n, const int *dim, const int *dim1, const int *dim2);
+/* multiply1/multiply2 (:762-807): n = 0 means every component times
+ * value[0]; otherwise this(dim(i)) = a1(dim1(i)) * value(i) */
+int qp_field2d_mult(qp_field2d *f, const qp_field2d *a1, int n,
+                    const int *dim, const int *dim1, const double *value);
+int qp_field2d_ag(qp_field2d *f);                                  /* :454 */
+int qp_field2d_cg(qp_field2d *f);                                  /* :434 */
+/* copyto :538-560: 3-D(ddim(i), lpos) = this(sdim(i)) */
+int qp_field2d_cb(const qp_field2d *f, qp_field3d *f3, int lpos, int n,
+                  const int *sdim, const int *ddim);
+/* copyadd :586-610: this(ddim(i)) += 3-D(sdim(i), lpos) */
+int qp_field2d_ca(qp_field2d *f, const qp_field3d *f3, int lpos, int n,
+                  const int *sdim, const int *ddim);
+
+/* ---------------------------------------------------------------------- */
+/* field3d (field3d_class.f03, ufield3d_class.f03:74-120)                  */
+/* ---------------------------------------------------------------------- */
+int qp_field3d_create(qp_ctx *ctx, int dim, qp_field3d **out);
+int qp_field3d_destroy(qp_field3d *f);
+int qp_field3d_as(qp_field3d *f, double value);
+int qp_field3d_upload(qp_field3d *f, const double *host, int ld1, int ld2,
+                      int ld3);
+int qp_field3d_download(const qp_field3d *f, double *host, int ld1, int ld2,
+                        int ld3);
+/* one xi plane (lpos 1-based) of one component, real layout (ld1 >= nx+1) */
+int qp_field3d_download_plane(const qp_field3d *f, int comp, int lpos,
+                              double *host, int ld1, int ld2);
+
+/* ---------------------------------------------------------------------- */
+/* species2d (species2d_class.f03) with the uniform profile 0             */
+/* (fdist2d_class.f03:168-316)                                             */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    int ppc[2];       /* particles per cell in x, y */
+    double density;   /* n/n0 */
+    double q, m;      /* charge and mass (qbm = q/m) */
+} qp_species_params;
+
+/* init_species2d :74-118; dt is the xi step (simulation_class.f03:578) */
+int qp_species_create(qp_ctx *ctx, const qp_species_params *p, double dt,
+                      double s, qp_species **out);
+int qp_species_destroy(qp_species *sp);
+int qp_species_renew(qp_species *sp, double s);                   /* :138 */
+int qp_species_qdp(qp_species *sp, qp_field2d *q);                /* :166 */
+int qp_species_extpsi(qp_species *sp, const qp_field2d *psi, double dex);
+int qp_species_amjdp(qp_species *sp, const qp_field2d *ef,
+                     const qp_field2d *bf, const qp_field2d *psit,
+                     qp_field2d *cu, qp_field2d *amu, qp_field2d *dcu,
+                     double dex);                                  /* :187 */
+int qp_species_push(qp_species *sp, const qp_field2d *ef,
+                    const qp_field2d *bf, const qp_field2d *psit,
+                    double dex);                                   /* :220 */
+int qp_species_cbq(qp_species *sp, qp_field3d *q3, int pos);      /* :399 */
+int qp_species_count(const qp_species *sp, int64_t *np);
+/* particles in the library's cell order, p(8, ld >= np) */
+int qp_species_download(const qp_species *sp, double *part, int64_t ld);
+int qp_species_upload(qp_species *sp, const double *part, int64_t np,
+                      int64_t ld);
+/* per-cell counts, cell = y*nx + x (ny*nx values) */
+int qp_species_cell_counts(const qp_species *sp, int32_t *counts);
+/* per-tile counts in the reference's 16x16 tile order (kpic, PPDBLKP2L) */
+int qp_species_tile_counts(const qp_species *sp, int32_t *kpic);
+
+/* ---------------------------------------------------------------------- */
+/* beam3d (beam3d_class.f03) with the tri-Gaussian profile 0              */
+/* (fdist3d_class.f03:222-421, PRVDIST32_RANDOM part3d_lib77.f:411-552)    */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    double center[3];      /* c/wp, in box coordinates */
+    double sigma[3];       /* c/wp */
+    double sigma_v[3];     /* mc */
+    double centroid_x[3];  /* centroid polynomial coefficients */
+    double centroid_y[3];
+    double gamma, peak, q, m;
+    int np[3];             /* npx, npy, npz */
+    int quiet;
+    int evolution;         /* 0: the beam is not pushed (beam3d_class.f03:170) */
+    int64_t npmax;
+} qp_beam_params;
+
+/* init_beam3d :61-93; seed 0 is the reference's fixed ranorm state,
+ * other seeds start independent streams */
+int qp_beam_create(qp_ctx *ctx, const qp_beam_params *p, double dt,
+                   uint64_t seed, qp_beam **out);
+int qp_beam_destroy(qp_beam *b);
+int qp_beam_qdp(qp_beam *b);                                       /* :111 */
+int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice);         /* :136 */
+int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf,
+                 double dex, double dez);                          /* :155 */
+int qp_beam_count(const qp_beam *b, int64_t *np);
+int qp_beam_download(const qp_beam *b, double *part, int64_t ld);
+int qp_beam_upload(qp_beam *b, const double *part, int64_t np, int64_t ld);
+int qp_beam_charge_download(const qp_beam *b, double *host, int ld1, int ld2,
+                            int ld3);
+/* sum of gamma and of the beam charge (FP64 reductions) */
+int qp_beam_moments(const qp_beam *b, double *sum_gamma, double *sum_q);
+
+/* ---------------------------------------------------------------------- */
+/* the 3-D step and its xi-slice loop (simulation_class.f03:611-765)      */
+/* ---------------------------------------------------------------------- */
+/* which fields qp_sim_field2d returns */
+enum { QP_F_QB = 0, QP_F_QE, QP_F_PSIT, QP_F_PSI, QP_F_DIVVPOT, QP_F_FXY,
+       QP_F_BXYZ, QP_F_CU, QP_F_DCU, QP_F_AMU, QP_F_EPW, QP_F_EPWB,
+       QP_F_COUNT };
+
+int qp_sim_create(qp_ctx *ctx, qp_species *const *sp, int nsp,
+                  qp_beam *const *bm, int nb, int iter, qp_sim **out);
+int qp_sim_destroy(qp_sim *sim);
+int qp_sim_field2d(qp_sim *sim, int which, qp_field2d **out);
+/* bexyz (Ex, Ey, Ez) and bbxyz (Bx, By, Bz) 3-D wake arrays */
+int qp_sim_field3d(qp_sim *sim, qp_field3d **bexyz, qp_field3d **bbxyz);
+/* optional target of the species charge diagnostic cbq (species index from
+ * 0 in the order given to qp_sim_create); without one cbq still runs */
+int qp_sim_species_q3(qp_sim *sim, int species, qp_field3d *q3);
+/* start of a 3-D step: beam deposits, zero 2-D fields (pipeline receive on
+ * the first stage, ufield2d_class.f03:528-531), slice-1 copies (:639-641) */
+int qp_sim_begin_step(qp_sim *sim);
+/* one slice j (1-based).  tol <= 0: exactly iter_max corrector passes, as
+ * the reference (iter_max = iter).  tol > 0: passes until the relative
+ * change of B_perp in k space drops below tol, at most iter_max.  Reports
+ * the passes used and the last relative change (FP64). */
+int qp_slice_run(qp_sim *sim, int j, int iter_max, double tol,
+                 int *iters_used, double *residual);
+/* end of the 3-D step: beam push (:752-757) and species renew (:761-764) */
+int qp_sim_end_step(qp_sim *sim, double s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QPSLICE_H */

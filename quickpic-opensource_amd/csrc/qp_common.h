@@ -1,0 +1,142 @@
+// Internal declarations of libqpslice.so (MI355X / gfx950).
+//
+// Device layouts (row-major, one plane per component):
+//   real-space plane  (ny+1) rows x ldr   (row y holds nodes x = 0..nx)
+//   k-space plane     (nx+1) rows x ldk   (row kx holds modes ky = 0..ny;
+//                     the reference's transposed k order, ufield2d_class.f03:198)
+//   3-D stack         (nz+1) real-space planes per component
+//   particles         structure of arrays, sorted by a tile-major cell key
+//                     key = (ty*mx1 + tx)*256 + ly*16 + lx for cell
+//                     (x, y) = (16*tx + lx, 16*ty + ly)
+// Real is float (QP_FP32) or double (QP_FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/qpslice.h"
+
+namespace qp {
+
+constexpr int TILE = 16;                 // cells per tile edge (part2d_class.f03:99)
+constexpr int TCELLS = TILE * TILE;      // cells per tile = threads per particle block
+constexpr int HALO = 4 * TILE;           // border nodes of a 17x17 tile block
+constexpr double TWOPI_REF = 6.28318530717959;  // the reference's 2*pi literal
+constexpr double BEAM_FIX = 68719476736.0;      // 2^36: beam charge fixed point
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define QP_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) return qp::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define QP_LAUNCH_CHECK()                                                   \
+    do {                                                                    \
+        hipError_t _e = hipGetLastError();                                  \
+        if (_e != hipSuccess) return qp::hip_fail(_e, "kernel launch", __FILE__, __LINE__); \
+    } while (0)
+
+#define QP_REQUIRE(cond, code, ...)                                         \
+    do {                                                                    \
+        if (!(cond)) { qp::set_error(__VA_ARGS__); return (code); }         \
+    } while (0)
+
+template <typename T> struct cplx { T x, y; };
+
+// device buffer owned by the library
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int alloc(size_t n);
+    void release();
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace qp
+
+namespace qp {
+struct ProfEv {
+    hipEvent_t a, b;
+    int kind;
+};
+}  // namespace qp
+
+struct qp_ctx {
+    int indx, indy, indz, nx, ny, nz;
+    int prec;             // QP_FP64 / QP_FP32
+    int device;
+    size_t esize;         // sizeof(Real)
+    double box[6];
+    double dex, dey, dxi;
+    int ldr, ldk;         // row pitches of real / k planes (elements)
+    size_t rplane, kplane;  // elements per real / k plane
+    int mx1, my1, ntiles, ncells;
+    hipStream_t stream;
+    qp::DevBuf ffg, ffs;          // Green's function and shape factor [kx][ky], nx*ny
+    qp::DevBuf twx, twy;          // twiddles exp(-2 pi i t / 2n), t < 2n
+    qp::DevBuf scr_r, scr_k;      // transform scratch, 3 planes each
+    qp::DevBuf halo;              // deposit tile borders, 8 comps
+    qp::DevBuf red;               // reduction partials (double)
+    size_t max_lds;               // LDS bytes one workgroup may use
+    // kernel-class timing (qp_ctx_profile)
+    int prof_on;
+    std::vector<qp::ProfEv> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[QP_PROF_COUNT];
+    int64_t prof_n[QP_PROF_COUNT];
+};
+
+struct qp_field2d {
+    qp_ctx *ctx;
+    int dim;
+    int state;   // 0 real, 1 k
+    int gcells;
+    qp::DevBuf rs, ks;
+};
+
+struct qp_field3d {
+    qp_ctx *ctx;
+    int dim;
+    qp::DevBuf buf;   // dim * (nz+1) real planes
+};
+
+namespace qp {
+
+// ---- transforms (qp_fft.hip) ----
+// component (x, y) transform types of fftrk/fftkr kinds; 0 = sine, 1 = cosine
+void kind_types(int kind, int dim, int *tx, int *ty);
+int fft_r2k(qp_field2d *f, int kind);
+int fft_k2r(qp_field2d *f, int kind);
+
+// ---- field kernels (qp_field.hip) ----
+int launch_fill(qp_ctx *c, void *p, size_t n, double v);
+int copyguard(qp_field2d *f);
+int guard_fold(qp_ctx *c, void *planes, int dim);
+template <typename R> int bfqp_resid(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
+                                     qp_field2d *bxy, double c, double dex, double *num_den);
+int bfqp_impl(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy,
+              double c, double dex, double *resid_out);
+
+// ---- kernel-class timing ----
+int prof_begin(qp_ctx *c, int kind);         // returns a slot or -1
+void prof_end(qp_ctx *c, int slot);
+
+struct ProfScope {
+    qp_ctx *c;
+    int slot;
+    ProfScope(qp_ctx *cc, int kind) : c(cc), slot(cc->prof_on ? prof_begin(cc, kind) : -1) {}
+    ~ProfScope() { if (slot >= 0) prof_end(c, slot); }
+};
+
+// ---- species internals used by the slice driver (qp_particles.hip) ----
+int species_qdp_into(qp_species *sp, qp_field2d *q);
+
+}  // namespace qp

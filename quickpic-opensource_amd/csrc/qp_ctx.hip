@@ -1,0 +1,265 @@
+// Context, device buffers, error reporting and the solver tables.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "qp_common.h"
+
+namespace qp {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e),
+              what, file, line);
+    return QP_ERR_HIP;
+}
+
+int DevBuf::alloc(size_t n)
+{
+    release();
+    if (n == 0) return QP_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+        return QP_ERR_ALLOC;
+    }
+    bytes = n;
+    e = hipMemset(p, 0, n);
+    if (e != hipSuccess) return hip_fail(e, "hipMemset", __FILE__, __LINE__);
+    return QP_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+static size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Poisson table of MPPOISD22 with isign = 0 (fpois2d_lib77.f:1514-1531):
+// g = affp*s/k^2 (affp at k = 0), s = exp(-((kx ax)^2 + (ky ay)^2)/2),
+// kx = pi j/nx, ky = pi k/ny, with the reference's 2*pi literal.  Computed in
+// double in the reference's operation order, then stored as Real.
+static void pois_table(int nx, int ny, std::vector<double> &g, std::vector<double> &s)
+{
+    const double ax = 0.912871, ay = 0.912871, affp = 1.0;  // field2d_class.f03:129
+    const double dnx = TWOPI_REF / (double)(nx + nx);
+    const double dny = TWOPI_REF / (double)(ny + ny);
+    g.resize((size_t)nx * ny);
+    s.resize((size_t)nx * ny);
+    for (int j = 0; j < nx; ++j) {
+        double dkx = dnx * (double)j;
+        double at1 = dkx * dkx;
+        double at2 = (dkx * ax) * (dkx * ax);
+        for (int k = 0; k < ny; ++k) {
+            double dky = dny * (double)k;
+            double at3 = dky * dky + at1;
+            double at4 = std::exp(-0.5 * ((dky * ay) * (dky * ay) + at2));
+            size_t o = (size_t)j * ny + k;
+            if (at3 == 0.0) { g[o] = affp; s[o] = 1.0; }
+            else { g[o] = affp * at4 / at3; s[o] = at4; }
+        }
+    }
+}
+
+template <typename R>
+static int upload_real(DevBuf &b, const std::vector<double> &v)
+{
+    std::vector<R> h(v.begin(), v.end());
+    int rc = b.alloc(h.size() * sizeof(R));
+    if (rc) return rc;
+    QP_HIP(hipMemcpy(b.p, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice));
+    return QP_OK;
+}
+
+// twiddles W[t] = exp(-2 pi i t / m), t = 0..m-1, accurate to double
+template <typename R>
+static int upload_twiddles(DevBuf &b, int m)
+{
+    std::vector<cplx<R>> h((size_t)m);
+    for (int t = 0; t < m; ++t) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * t / m;
+        h[t].x = (R)cosl(a);
+        h[t].y = (R)(-sinl(a));
+    }
+    int rc = b.alloc(h.size() * sizeof(cplx<R>));
+    if (rc) return rc;
+    QP_HIP(hipMemcpy(b.p, h.data(), h.size() * sizeof(cplx<R>), hipMemcpyHostToDevice));
+    return QP_OK;
+}
+
+template <typename R>
+static int make_tables(qp_ctx *c)
+{
+    std::vector<double> g, s;
+    pois_table(c->nx, c->ny, g, s);
+    int rc = upload_real<R>(c->ffg, g);
+    if (!rc) rc = upload_real<R>(c->ffs, s);
+    if (!rc) rc = upload_twiddles<R>(c->twx, 2 * c->nx);
+    if (!rc) rc = upload_twiddles<R>(c->twy, 2 * c->ny);
+    return rc;
+}
+
+static hipEvent_t pool_get(qp_ctx *c)
+{
+    if (!c->prof_pool.empty()) {
+        hipEvent_t e = c->prof_pool.back();
+        c->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int prof_begin(qp_ctx *c, int kind)
+{
+    ProfEv ev{pool_get(c), pool_get(c), kind};
+    if (!ev.a || !ev.b) return -1;
+    if (hipEventRecord(ev.a, c->stream) != hipSuccess) return -1;
+    c->prof_pending.push_back(ev);
+    return (int)c->prof_pending.size() - 1;
+}
+
+void prof_end(qp_ctx *c, int slot)
+{
+    (void)hipEventRecord(c->prof_pending[slot].b, c->stream);
+}
+
+static int prof_resolve(qp_ctx *c)
+{
+    QP_HIP(hipStreamSynchronize(c->stream));
+    for (ProfEv &e : c->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            c->prof_ms[e.kind] += ms;
+            c->prof_n[e.kind] += 1;
+        }
+        c->prof_pool.push_back(e.a);
+        c->prof_pool.push_back(e.b);
+    }
+    c->prof_pending.clear();
+    return QP_OK;
+}
+
+}  // namespace qp
+
+using namespace qp;
+
+extern "C" {
+
+const char *qp_last_error(void) { return qp::g_err; }
+
+int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precision,
+                  int device, qp_ctx **out)
+{
+    QP_REQUIRE(out && box, QP_ERR_ARG, "qp_ctx_create: null argument");
+    QP_REQUIRE(indx >= 4 && indy >= 4 && indx <= 12 && indy <= 12, QP_ERR_UNSUPPORTED,
+               "qp_ctx_create: transverse grid must be 2^4..2^12 (got 2^%d x 2^%d)", indx, indy);
+    QP_REQUIRE(indz >= 1 && indz <= 13, QP_ERR_ARG, "qp_ctx_create: bad indz %d", indz);
+    QP_REQUIRE(precision == QP_FP64 || precision == QP_FP32, QP_ERR_ARG,
+               "qp_ctx_create: bad precision %d", precision);
+    QP_HIP(hipSetDevice(device));
+    qp_ctx *c = new qp_ctx();
+    c->indx = indx; c->indy = indy; c->indz = indz;
+    c->nx = 1 << indx; c->ny = 1 << indy; c->nz = 1 << indz;
+    c->prec = precision;
+    c->device = device;
+    c->esize = precision == QP_FP64 ? 8 : 4;
+    std::memcpy(c->box, box, sizeof(c->box));
+    // simulation_class.f03:174-188: dx = (max-min)/2^indx etc.
+    c->dex = (box[1] - box[0]) / (double)(1 << indx);
+    c->dey = (box[3] - box[2]) / (double)(1 << indy);
+    c->dxi = (box[5] - box[4]) / (double)(1 << indz);
+    c->ldr = (int)round_up((size_t)c->nx + 1, 64);
+    c->ldk = (int)round_up((size_t)c->ny + 1, 64);
+    c->rplane = (size_t)(c->ny + 1) * c->ldr;
+    c->kplane = (size_t)(c->nx + 1) * c->ldk;
+    c->mx1 = c->nx / TILE;
+    c->my1 = c->ny / TILE;
+    c->ntiles = c->mx1 * c->my1;
+    c->ncells = c->nx * c->ny;
+    int lds = 0;
+    hipError_t e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    c->max_lds = (e == hipSuccess && lds > 0) ? (size_t)lds : 65536;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
+    int rc = precision == QP_FP64 ? make_tables<double>(c) : make_tables<float>(c);
+    size_t pl = c->rplane > c->kplane ? c->rplane : c->kplane;
+    if (!rc) rc = c->scr_r.alloc(3 * pl * c->esize);
+    if (!rc) rc = c->scr_k.alloc(3 * pl * c->esize);
+    if (!rc) rc = c->halo.alloc((size_t)8 * c->ntiles * HALO * c->esize);
+    if (!rc) rc = c->red.alloc(sizeof(double) * 4096);
+    if (rc) { qp_ctx_destroy(c); return rc; }
+    *out = c;
+    return QP_OK;
+}
+
+int qp_ctx_destroy(qp_ctx *c)
+{
+    if (!c) return QP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->ffg.release(); c->ffs.release(); c->twx.release(); c->twy.release();
+    c->scr_r.release(); c->scr_k.release(); c->halo.release(); c->red.release();
+    for (ProfEv &e : c->prof_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return QP_OK;
+}
+
+int qp_ctx_profile(qp_ctx *c, int enable)
+{
+    QP_REQUIRE(c, QP_ERR_ARG, "qp_ctx_profile: null context");
+    int rc = prof_resolve(c);
+    if (rc) return rc;
+    for (int k = 0; k < QP_PROF_COUNT; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    c->prof_on = enable ? 1 : 0;
+    return QP_OK;
+}
+
+int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count)
+{
+    QP_REQUIRE(c && kind >= 0 && kind < QP_PROF_COUNT, QP_ERR_ARG, "qp_ctx_profile_get: bad argument");
+    int rc = prof_resolve(c);
+    if (rc) return rc;
+    if (total_ms) *total_ms = c->prof_ms[kind];
+    if (count) *count = c->prof_n[kind];
+    return QP_OK;
+}
+
+int qp_ctx_sync(qp_ctx *c)
+{
+    QP_REQUIRE(c, QP_ERR_ARG, "qp_ctx_sync: null context");
+    QP_HIP(hipStreamSynchronize(c->stream));
+    return QP_OK;
+}
+
+int qp_ctx_info(const qp_ctx *c, int *nx, int *ny, int *nz, int *precision, double *dex,
+                double *dxi)
+{
+    QP_REQUIRE(c, QP_ERR_ARG, "qp_ctx_info: null context");
+    if (nx) *nx = c->nx;
+    if (ny) *ny = c->ny;
+    if (nz) *nz = c->nz;
+    if (precision) *precision = c->prec;
+    if (dex) *dex = c->dex;
+    if (dxi) *dxi = c->dxi;
+    return QP_OK;
+}
+
+}  // extern "C"

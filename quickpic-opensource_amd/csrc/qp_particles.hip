@@ -1,0 +1,908 @@
+// Plasma species on the GPU: initial load, sort by cell, charge deposit,
+// psi gather, current/acceleration/momentum-flux deposit and the push.
+//
+// Particles are a structure of arrays sorted by a tile-major cell key (see
+// qp_common.h).  Every particle kernel is cell-centric: one workgroup per
+// 16x16-cell tile and one thread per cell.  A thread reads the four corner
+// values of the fields its cell needs once, then walks the cell's particles
+// (a contiguous run of the sorted arrays), so the per-particle gathers need no
+// memory traffic and the per-particle deposits accumulate in registers.  The
+// four corner sums of every cell are combined into the tile's 17x17 nodes in
+// LDS in a fixed order; nodes inside the tile are written directly and the 64
+// border nodes go to a halo buffer that a second kernel sums, again in a fixed
+// order, with the one-rank guard fold of PPNACGUARD2L
+// (ufield2d_lib.f03:90-97).  No atomics touch the deposits.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+
+#include "qp_common.h"
+
+struct qp_species {
+    qp_ctx *ctx;
+    qp_species_params p;
+    double qbm, dt;
+    int64_t np;
+    qp::DevBuf a[8], b[8];     // x, y, px, py, pz, p6, p7, q (current, alternate)
+    qp::DevBuf key, slot;      // per particle: destination cell key and rank
+    qp::DevBuf count, off;     // per cell (ncells + 1)
+    qp::DevBuf scan_tmp, err;
+    size_t scan_bytes;
+    qp_field2d *q, *qn, *cu, *dcu, *amu;
+};
+
+namespace qp {
+
+template <typename R> struct Soa {
+    R *x, *y, *px, *py, *pz, *p6, *p7, *q;
+};
+
+template <typename R> static Soa<R> soa(DevBuf *a)
+{
+    return Soa<R>{a[0].as<R>(), a[1].as<R>(), a[2].as<R>(), a[3].as<R>(),
+                  a[4].as<R>(), a[5].as<R>(), a[6].as<R>(), a[7].as<R>()};
+}
+
+struct TileGeom {
+    int nx, ny, mx1, my1, ldr;
+};
+
+__device__ __forceinline__ int cell_key(int cx, int cy, int mx1)
+{
+    return ((cy >> 4) * mx1 + (cx >> 4)) * TCELLS + ((cy & 15) << 4) + (cx & 15);
+}
+
+// this thread's cell: key = blockIdx.x*256 + threadIdx.x
+__device__ __forceinline__ void my_cell(const TileGeom &g, int &cx, int &cy)
+{
+    int t = blockIdx.x, l = threadIdx.x;
+    cx = (t % g.mx1) * TILE + (l & 15);
+    cy = (t / g.mx1) * TILE + (l >> 4);
+}
+
+template <typename R> struct Corners {
+    R f00, f10, f01, f11;
+};
+
+template <typename R>
+__device__ __forceinline__ Corners<R> load_corners(const R *plane, int ld, int cx, int cy)
+{
+    const R *r0 = plane + (size_t)cy * ld + cx, *r1 = r0 + ld;
+    return Corners<R>{r0[0], r0[1], r1[0], r1[1]};
+}
+
+// bilinear gather in the reference's order (e.g. part2d_lib77.f:2265-2268)
+template <typename R>
+__device__ __forceinline__ R gather(const Corners<R> &f, R amx, R amy, R dxp, R dyp)
+{
+    R d = amx * f.f00;
+    d = amy * (dxp * f.f10 + d);
+    R a = amx * f.f01;
+    return d + dyp * (dxp * f.f11 + a);
+}
+
+__device__ __forceinline__ int halo_index(int i, int j)
+{
+    if (j == 0) return i;
+    if (j == TILE) return 17 + i;
+    if (i == 0) return 34 + (j - 1);
+    return 49 + (j - 1);
+}
+
+// Combine this thread's cell-corner sums of one component into the tile's
+// nodes: interior nodes straight to the output plane, border nodes to halo.
+template <typename R>
+__device__ __forceinline__ void emit_tile(R c00, R c10, R c01, R c11, R *lds, R *plane, int ld,
+                                          R *halo, int ntiles)
+{
+    const int tid = threadIdx.x, t = blockIdx.x;
+    lds[tid] = c00;
+    lds[TCELLS + tid] = c10;
+    lds[2 * TCELLS + tid] = c01;
+    lds[3 * TCELLS + tid] = c11;
+    __syncthreads();
+    (void)ntiles;
+    for (int n = tid; n < 17 * 17; n += TCELLS) {
+        const int i = n % 17, j = n / 17;
+        R s = R(0);
+        if (i < TILE && j < TILE) s += lds[j * TILE + i];
+        if (i > 0 && j < TILE) s += lds[TCELLS + j * TILE + i - 1];
+        if (i < TILE && j > 0) s += lds[2 * TCELLS + (j - 1) * TILE + i];
+        if (i > 0 && j > 0) s += lds[3 * TCELLS + (j - 1) * TILE + i - 1];
+        if (i >= 1 && i < TILE && j >= 1 && j < TILE)
+            plane[(size_t)(j) * ld + i] = s;
+        else
+            halo[(size_t)t * HALO + halo_index(i, j)] = s;
+    }
+    __syncthreads();
+}
+
+template <typename R> struct PlaneSet {
+    R *p[8];
+};
+
+// Sum the tile-border contributions of every border node in a fixed order;
+// with fold, row ny is added into row 0 and cleared (PPNACGUARD2L nvp = 1).
+template <typename R>
+__global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t halo_cs,
+                                 PlaneSet<R> out, int fold)
+{
+    const int comp = blockIdx.y;
+    const int rows_n = (g.my1 + 1) * (g.nx + 1);
+    const int cols_n = (g.mx1 + 1) * g.my1 * (TILE - 1);
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    int X, Y;
+    if (id < rows_n) {
+        Y = (id / (g.nx + 1)) * TILE;
+        X = id % (g.nx + 1);
+    } else if (id < rows_n + cols_n) {
+        int k = id - rows_n;
+        int per_col = g.my1 * (TILE - 1);
+        X = (k / per_col) * TILE;
+        int r = k % per_col;
+        Y = (r / (TILE - 1)) * TILE + 1 + r % (TILE - 1);
+    } else {
+        return;
+    }
+    const R *h = halo + comp * halo_cs;
+    auto node_sum = [&](int XX, int YY) -> R {
+        R s = R(0);
+        for (int tyc = (YY >> 4) - 1; tyc <= (YY >> 4); ++tyc) {
+            if (tyc < 0 || tyc >= g.my1) continue;
+            int ly = YY - TILE * tyc;
+            if (ly < 0 || ly > TILE) continue;
+            for (int txc = (XX >> 4) - 1; txc <= (XX >> 4); ++txc) {
+                if (txc < 0 || txc >= g.mx1) continue;
+                int lx = XX - TILE * txc;
+                if (lx < 0 || lx > TILE) continue;
+                s += h[(size_t)(tyc * g.mx1 + txc) * HALO + halo_index(lx, ly)];
+            }
+        }
+        return s;
+    };
+    R v;
+    if (fold && Y == g.ny) v = R(0);
+    else if (fold && Y == 0) { v = node_sum(X, 0); v = v + node_sum(X, g.ny); }
+    else v = node_sum(X, Y);
+    out.p[comp][(size_t)Y * g.ldr + X] = v;
+}
+
+// PPGPPOST2L (part2d_lib77.f:194-309): q(n,m) += qm (1-dx)(1-dy) etc.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, R *plane, R *halo)
+{
+    __shared__ R lds[4 * TCELLS];
+    int cx, cy;
+    my_cell(g, cx, cy);
+    const int key = blockIdx.x * TCELLS + threadIdx.x;
+    const int i0 = off[key], i1 = off[key + 1];
+    const R fx = (R)cx, fy = (R)cy;
+    R a00 = 0, a10 = 0, a01 = 0, a11 = 0;
+    for (int i = i0; i < i1; ++i) {
+        R x = s.x[i], y = s.y[i], qm = s.q[i];
+        R dxp = qm * (x - fx);
+        R dyp = y - fy;
+        R amx = qm - dxp;
+        R amy = R(1) - dyp;
+        a00 += amx * amy;
+        a10 += dxp * amy;
+        a01 += amx * dyp;
+        a11 += dxp * dyp;
+    }
+    const int tx = blockIdx.x % g.mx1, ty = blockIdx.x / g.mx1;
+    emit_tile<R>(a00, a10, a01, a11, lds, plane + (size_t)(ty * TILE) * g.ldr + tx * TILE, g.ldr,
+                 halo, g.mx1 * g.my1);
+}
+
+// WPGPSIPOST2L_QP (part2d_lib77.f:2207-2284)
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_extpsi(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *psi, R qbm, R dex)
+{
+    int cx, cy;
+    my_cell(g, cx, cy);
+    const int key = blockIdx.x * TCELLS + threadIdx.x;
+    const int i0 = off[key], i1 = off[key + 1];
+    if (i0 == i1) return;
+    const Corners<R> f = load_corners(psi, g.ldr, cx, cy);
+    const R dx2 = dex * dex, fx = (R)cx, fy = (R)cy;
+    for (int i = i0; i < i1; ++i) {
+        R dxp = s.x[i] - fx, dyp = s.y[i] - fy;
+        R amx = R(1) - dxp, amy = R(1) - dyp;
+        R vx = s.px[i], vy = s.py[i];
+        R dx = gather(f, amx, amy, dxp, dyp);
+        dx = -(dx * dx2 * qbm);
+        s.p7[i] = R(1) + dx;
+        s.p6[i] = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
+    }
+}
+
+// PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768): half Boris step with the B_z
+// rotation, deposit of J (3), dJ/dxi (2) and momentum flux (3); p6 updated.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
+        const R *psit, size_t cs, PlaneSet<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex)
+{
+    __shared__ R lds[4 * TCELLS];
+    int cx, cy;
+    my_cell(g, cx, cy);
+    const int key = blockIdx.x * TCELLS + threadIdx.x;
+    const int i0 = off[key], i1 = off[key + 1];
+    R acc[8][4];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
+    if (i1 > i0) {
+        const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
+        const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
+        const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
+        const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
+        const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
+        const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
+        const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
+        const R fx = (R)cx, fy = (R)cy;
+        for (int i = i0; i < i1; ++i) {
+            R p6 = s.p6[i], p7 = s.p7[i], qm = s.q[i];
+            R dxp = s.x[i] - fx, dyp = s.y[i] - fy;
+            R amx = R(1) - dxp, amy = R(1) - dyp;
+            R inv_part_7 = R(1) / p7;
+            R qtmh1 = qtmh * inv_part_7;
+            R qtmh2 = qtmh1 * p6;
+            R dx = gather(ex, amx, amy, dxp, dyp);
+            R dy = gather(ey, amx, amy, dxp, dyp);
+            R dz = gather(ez, amx, amy, dxp, dyp);
+            R ox = gather(bx, amx, amy, dxp, dyp);
+            R oy = gather(by, amx, amy, dxp, dyp);
+            R oz = gather(bz, amx, amy, dxp, dyp);
+            R ddx = (R(-1)) * qtmh2 * dx + qtmh * oy;
+            R ddy = (R(-1)) * qtmh2 * dy - qtmh * ox;
+            R vx = s.px[i], vy = s.py[i];
+            R acx = vx + ddx, acy = vy + ddy;
+            R omzt = qtmh1 * oz;
+            R omt = omzt * omzt;
+            R anorm = R(2) / (R(1) + omt);
+            R rot1 = R(0.5) * (R(1) - omt);
+            R rot2 = omzt;
+            R v1 = (rot1 * acx + rot2 * acy) * anorm + ddx;
+            R v2 = (rot1 * acy - rot2 * acx) * anorm + ddy;
+            R qm1 = qm * inv_part_7;
+            amx = qm1 * amx;
+            dxp = qm1 * dxp;
+            ox = R(0.5) * (v1 + vx);
+            oy = R(0.5) * (v2 + vy);
+            oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
+            s.p6[i] = p7 + oz;
+            oz = oz * idex;
+            vx = (v1 - vx) * dti;
+            vy = (v2 - vy) * dti;
+            dz = qbm * (dz + (dx * ox + dy * oy) * dex * dex * inv_part_7);
+            R w[4] = {amx * amy, dxp * amy, amx * dyp, dxp * dyp};
+            v1 = ox * ox * inv_part_7;
+            v2 = oy * oy * inv_part_7;
+            R v3 = ox * oy * inv_part_7;
+            vx = vx + ox * dz * inv_part_7;
+            vy = vy + oy * dz * inv_part_7;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc[0][k] += ox * w[k];
+                acc[1][k] += oy * w[k];
+                acc[2][k] += oz * w[k];
+                acc[3][k] += vx * w[k];
+                acc[4][k] += vy * w[k];
+                acc[5][k] += v1 * w[k];
+                acc[6][k] += v2 * w[k];
+                acc[7][k] += v3 * w[k];
+            }
+        }
+    }
+    const int tx = blockIdx.x % g.mx1, ty = blockIdx.x / g.mx1;
+    const size_t base = (size_t)(ty * TILE) * g.ldr + tx * TILE;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        emit_tile<R>(acc[c][0], acc[c][1], acc[c][2], acc[c][3], lds, out.p[c] + base, g.ldr,
+                     halo + c * halo_cs, g.mx1 * g.my1);
+}
+
+// PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
+// periodic wrap of the reorder (PPPORDER2LB :1936-1977).  Emits every
+// particle's new cell key and its rank in that cell.  A particle whose new
+// tile is not its old tile plus the leaving direction (the condition
+// PPPCHECK2L reports) sets *err.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
+       const R *psit, size_t cs, R qbm, R dt, R dex, int *newkey, int *slot, int *count, int *err)
+{
+    int cx, cy;
+    my_cell(g, cx, cy);
+    const int key = blockIdx.x * TCELLS + threadIdx.x;
+    const int i0 = off[key], i1 = off[key + 1];
+    if (i0 == i1) return;
+    const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
+    const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
+    const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
+    const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
+    const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
+    const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
+    const R idex = R(1) / dex, qtmh = R(0.5) * qbm * dt;
+    const R anx = (R)g.nx, any = (R)g.ny;
+    const R fx = (R)cx, fy = (R)cy;
+    const int tx0 = cx >> 4, ty0 = cy >> 4;
+    const R edgelx = (R)(tx0 * TILE), edgely = (R)(ty0 * TILE);
+    const R edgerx = edgelx + (R)min(TILE, g.nx - tx0 * TILE);
+    const R edgery = edgely + (R)min(TILE, g.ny - ty0 * TILE);
+    for (int i = i0; i < i1; ++i) {
+        R x = s.x[i], y = s.y[i];
+        R p6 = s.p6[i], p7 = s.p7[i];
+        R qtmh1 = qtmh / p7;
+        R qtmh2 = qtmh1 * p6;
+        R dxp = x - fx, dyp = y - fy;
+        R amx = R(1) - dxp, amy = R(1) - dyp;
+        R dx = gather(ex, amx, amy, dxp, dyp);
+        R dy = gather(ey, amx, amy, dxp, dyp);
+        R dz = gather(ez, amx, amy, dxp, dyp);
+        dz = dz * idex;
+        R ox = gather(bx, amx, amy, dxp, dyp);
+        R oy = gather(by, amx, amy, dxp, dyp);
+        R oz = gather(bz, amx, amy, dxp, dyp);
+        ox = ox * dex;
+        oy = oy * dex;
+        dx = qtmh2 * dx;
+        dy = qtmh2 * dy;
+        dz = qtmh2 * dz;
+        R acx = s.px[i] + dx, acy = s.py[i] + dy, acz = s.pz[i] + dz;
+        R omxt = qtmh1 * ox, omyt = qtmh1 * oy, omzt = qtmh1 * oz;
+        R omt = omxt * omxt + omyt * omyt + omzt * omzt;
+        R anorm = R(2) / (R(1) + omt);
+        omt = R(0.5) * (R(1) - omt);
+        R rot4 = omxt * omyt, rot7 = omxt * omzt, rot8 = omyt * omzt;
+        R rot1 = omt + omxt * omxt;
+        R rot5 = omt + omyt * omyt;
+        R rot9 = omt + omzt * omzt;
+        R rot2 = omzt + rot4;
+        rot4 = -omzt + rot4;
+        R rot3 = -omyt + rot7;
+        rot7 = omyt + rot7;
+        R rot6 = omxt + rot8;
+        rot8 = -omxt + rot8;
+        dx = (rot1 * acx + rot2 * acy + rot3 * acz) * anorm + dx;
+        dy = (rot4 * acx + rot5 * acy + rot6 * acz) * anorm + dy;
+        dz = (rot7 * acx + rot8 * acy + rot9 * acz) * anorm + dz;
+        s.px[i] = dx;
+        s.py[i] = dy;
+        s.pz[i] = dz;
+        R dtc1 = dt / (sqrt(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
+        dx = x + dx * dtc1;
+        dy = y + dy * dtc1;
+        int dirx = 0, diry = 0;
+        if (dx >= edgerx) dirx = 1;
+        else if (dx < edgelx) {
+            dirx = -1;
+            if (dx < R(0) && dx + anx >= anx) { dx = R(0); dirx = 0; }
+        }
+        if (dy >= edgery) diry = 1;
+        else if (dy < edgely) {
+            diry = -1;
+            if (dy < R(0) && dy + any >= any) { dy = R(0); diry = 0; }
+        }
+        if (dx < R(0)) dx = dx + anx;
+        if (dx >= anx) dx = dx - anx;
+        if (dy < R(0)) dy = dy + any;
+        if (dy >= any) dy = dy - any;
+        s.x[i] = dx;
+        s.y[i] = dy;
+        int ncx = (int)dx, ncy = (int)dy;
+        // guard against a non-finite or out-of-range position before it is
+        // used as an index
+        if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
+            atomicMax(err, 1 << 30);
+            ncx = min(max(ncx, 0), g.nx - 1);
+            ncy = min(max(ncy, 0), g.ny - 1);
+        }
+        int txd = (tx0 + dirx + g.mx1) % g.mx1, tyd = (ty0 + diry + g.my1) % g.my1;
+        if ((ncx >> 4) != txd || (ncy >> 4) != tyd) atomicMax(err, 1 + txd + g.mx1 * tyd);
+        int k = cell_key(ncx, ncy, g.mx1);
+        newkey[i] = k;
+        slot[i] = atomicAdd(&count[k], 1);
+    }
+}
+
+// cell keys of unsorted particles (initial load and uploads)
+template <typename R>
+__global__ void k_keys(int64_t np, const R *x, const R *y, int nx, int ny, int mx1, int *key,
+                       int *slot, int *count, int *err)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    R xv = x[i], yv = y[i];
+    int cx = (int)xv, cy = (int)yv;
+    if (!(xv >= R(0) && yv >= R(0) && cx < nx && cy < ny)) {
+        atomicMax(err, 1 << 30);
+        cx = min(max(cx, 0), nx - 1);
+        cy = min(max(cy, 0), ny - 1);
+    }
+    int k = cell_key(cx, cy, mx1);
+    key[i] = k;
+    slot[i] = atomicAdd(&count[k], 1);
+}
+
+template <typename R>
+__global__ void k_scatter(int64_t np, Soa<R> a, Soa<R> b, const int *key, const int *slot,
+                          const int *off)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    int d = off[key[i]] + slot[i];
+    b.x[d] = a.x[i];
+    b.y[d] = a.y[i];
+    b.px[d] = a.px[i];
+    b.py[d] = a.py[i];
+    b.pz[d] = a.pz[i];
+    b.p6[d] = a.p6[i];
+    b.p7[d] = a.p7[i];
+    b.q[d] = a.q[i];
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+static TileGeom geom(qp_ctx *c) { return TileGeom{c->nx, c->ny, c->mx1, c->my1, c->ldr}; }
+
+static dim3 pgrid(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// counts -> offsets -> scatter into the alternate arrays, then swap
+template <typename R>
+static int rebin_tail(qp_species *sp)
+{
+    qp_ctx *c = sp->ctx;
+    size_t tb = sp->scan_bytes;
+    QP_HIP(hipcub::DeviceScan::ExclusiveSum(sp->scan_tmp.p, tb, sp->count.as<int>(),
+                                            sp->off.as<int>(), c->ncells + 1, c->stream));
+    {
+    ProfScope _ps(c, QP_PROF_SORT);
+    hipLaunchKernelGGL(k_scatter<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
+                       soa<R>(sp->a), soa<R>(sp->b), sp->key.as<int>(), sp->slot.as<int>(),
+                       sp->off.as<int>());
+    }
+    QP_LAUNCH_CHECK();
+    for (int k = 0; k < 8; ++k) std::swap(sp->a[k], sp->b[k]);
+    return QP_OK;
+}
+
+static int zero_counts(qp_species *sp)
+{
+    qp_ctx *c = sp->ctx;
+    hipLaunchKernelGGL(k_zero_int, dim3(1024), dim3(256), 0, c->stream, sp->count.as<int>(),
+                       c->ncells + 1);
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+static int check_err(qp_species *sp, const char *what)
+{
+    qp_ctx *c = sp->ctx;
+    int h = 0;
+    QP_HIP(hipMemcpyAsync(&h, sp->err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    QP_HIP(hipStreamSynchronize(c->stream));
+    if (h) {
+        QP_HIP(hipMemsetAsync(sp->err.p, 0, sizeof(int), c->stream));
+        if (h >= (1 << 30)) set_error("%s: particle position outside the grid", what);
+        else set_error("%s: particle left its tile by more than one tile (tile %d)", what, h);
+        return QP_ERR_CHECK;
+    }
+    return QP_OK;
+}
+
+template <typename R>
+static int rebin_unsorted(qp_species *sp)
+{
+    qp_ctx *c = sp->ctx;
+    int rc = zero_counts(sp);
+    if (rc) return rc;
+    {
+    ProfScope _ps(c, QP_PROF_SORT);
+    hipLaunchKernelGGL(k_keys<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
+                       sp->a[0].as<R>(), sp->a[1].as<R>(), c->nx, c->ny, c->mx1,
+                       sp->key.as<int>(), sp->slot.as<int>(), sp->count.as<int>(),
+                       sp->err.as<int>());
+    }
+    QP_LAUNCH_CHECK();
+    rc = rebin_tail<R>(sp);
+    if (rc) return rc;
+    return check_err(sp, "particle load");
+}
+
+template <typename R>
+static int deposit_border(qp_ctx *c, const PlaneSet<R> &out, int ncomp, int fold)
+{
+    TileGeom g = geom(c);
+    int n = (c->my1 + 1) * (c->nx + 1) + (c->mx1 + 1) * c->my1 * (TILE - 1);
+    {
+    ProfScope _ps(c, QP_PROF_BORDER);
+    hipLaunchKernelGGL(k_deposit_border<R>, dim3((n + 255) / 256, ncomp), dim3(256), 0, c->stream,
+                       g, c->halo.as<R>(), (size_t)c->ntiles * HALO, out, fold);
+    }
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+// species charge into a real-space plane (overwritten), guard-folded
+template <typename R>
+static int deposit_q_t(qp_species *sp, R *plane)
+{
+    qp_ctx *c = sp->ctx;
+    {
+    ProfScope _ps(c, QP_PROF_DEPOSIT_Q);
+    hipLaunchKernelGGL(k_deposit_q<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
+                       soa<R>(sp->a), sp->off.as<int>(), plane, c->halo.as<R>());
+    }
+    QP_LAUNCH_CHECK();
+    PlaneSet<R> ps{};
+    ps.p[0] = plane;
+    return deposit_border<R>(c, ps, 1, 1);
+}
+
+// dist2d_000 (fdist2d_class.f03:212-316), one rank: x in [1,nx-1),
+// y in [1,ny); record x, y, 0, 0, 0, 1, 1, qm in the reference's order.
+template <typename R>
+static int distribute(qp_species *sp)
+{
+    qp_ctx *c = sp->ctx;
+    const int nx = c->nx, ny = c->ny, xppc = sp->p.ppc[0], yppc = sp->p.ppc[1];
+    const int64_t np = (int64_t)(nx - 2) * (ny - 1) * xppc * yppc;
+    const double qm = 1.0 * sp->p.density * sp->p.q / std::fabs(sp->p.q) / (double)xppc / (double)yppc;
+    std::vector<R> hx((size_t)np), hy((size_t)np);
+    int64_t n = 0;
+    for (int i = 2; i <= nx - 1; ++i)
+        for (int j = 2; j <= ny; ++j)
+            for (int ix = 0; ix < xppc; ++ix)
+                for (int iy = 0; iy < yppc; ++iy) {
+                    hx[n] = (R)((ix + 0.5) / xppc + i - 1);
+                    hy[n] = (R)((iy + 0.5) / yppc + j - 1);
+                    ++n;
+                }
+    sp->np = np;
+    const size_t bytes = (size_t)np * sizeof(R);
+    QP_HIP(hipMemcpyAsync(sp->a[0].p, hx.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    QP_HIP(hipMemcpyAsync(sp->a[1].p, hy.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    QP_HIP(hipStreamSynchronize(c->stream));
+    const double fillv[6] = {0.0, 0.0, 0.0, 1.0, 1.0, qm};
+    for (int k = 2; k < 8; ++k) {
+        int rc = launch_fill(c, sp->a[k].p, (size_t)np, fillv[k - 2]);
+        if (rc) return rc;
+    }
+    return rebin_unsorted<R>(sp);
+}
+
+// renew: particles and qn = -(folded deposit of the fresh plasma)
+template <typename R>
+static int renew_t(qp_species *sp)
+{
+    int rc = distribute<R>(sp);
+    if (rc) return rc;
+    rc = deposit_q_t<R>(sp, sp->qn->rs.as<R>());
+    if (rc) return rc;
+    sp->qn->state = 0;
+    double m1 = -1.0;
+    return qp_field2d_mult(sp->qn, sp->qn, 0, nullptr, nullptr, &m1);
+}
+
+template <typename R>
+static int alloc_particles(qp_species *sp, int64_t np)
+{
+    qp_ctx *c = sp->ctx;
+    for (int k = 0; k < 8; ++k) {
+        int rc = sp->a[k].alloc((size_t)np * sizeof(R));
+        if (!rc) rc = sp->b[k].alloc((size_t)np * sizeof(R));
+        if (rc) return rc;
+    }
+    int rc = sp->key.alloc((size_t)np * 4);
+    if (!rc) rc = sp->slot.alloc((size_t)np * 4);
+    if (!rc) rc = sp->count.alloc((size_t)(c->ncells + 1) * 4);
+    if (!rc) rc = sp->off.alloc((size_t)(c->ncells + 1) * 4);
+    if (!rc) rc = sp->err.alloc(sizeof(int));
+    if (rc) return rc;
+    sp->scan_bytes = 0;
+    QP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sp->scan_bytes, sp->count.as<int>(),
+                                            sp->off.as<int>(), c->ncells + 1, c->stream));
+    return sp->scan_tmp.alloc(sp->scan_bytes + 256);
+}
+
+template <typename R>
+static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
+{
+    qp_ctx *c = sp->ctx;
+    {
+    ProfScope _ps(c, QP_PROF_EXTPSI);
+    hipLaunchKernelGGL(k_extpsi<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
+                       soa<R>(sp->a), sp->off.as<int>(), psi->rs.as<R>(), (R)sp->qbm, (R)dex);
+    }
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+template <typename R>
+static int amjdp_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                   const qp_field2d *psit, double dex)
+{
+    qp_ctx *c = sp->ctx;
+    PlaneSet<R> out{};
+    R *cu = sp->cu->rs.as<R>(), *dcu = sp->dcu->rs.as<R>(), *amu = sp->amu->rs.as<R>();
+    const size_t pl = c->rplane;
+    out.p[0] = cu; out.p[1] = cu + pl; out.p[2] = cu + 2 * pl;
+    out.p[3] = dcu; out.p[4] = dcu + pl;
+    out.p[5] = amu; out.p[6] = amu + pl; out.p[7] = amu + 2 * pl;
+    const size_t hcs = (size_t)c->ntiles * HALO;
+    {
+    ProfScope _ps(c, QP_PROF_AMJDP);
+    hipLaunchKernelGGL(k_amjdp<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
+                       soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
+                       psit->rs.as<R>(), pl, out, c->halo.as<R>(), hcs, (R)sp->qbm, (R)sp->dt,
+                       (R)dex);
+    }
+    QP_LAUNCH_CHECK();
+    int rc = deposit_border<R>(c, out, 8, 1);
+    sp->cu->state = sp->dcu->state = sp->amu->state = 0;
+    return rc;
+}
+
+template <typename R>
+static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                  const qp_field2d *psit, double dex, bool check)
+{
+    qp_ctx *c = sp->ctx;
+    int rc = zero_counts(sp);
+    if (rc) return rc;
+    {
+    ProfScope _ps(c, QP_PROF_PUSH);
+    hipLaunchKernelGGL(k_push<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
+                       soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
+                       psit->rs.as<R>(), c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex,
+                       sp->key.as<int>(), sp->slot.as<int>(), sp->count.as<int>(),
+                       sp->err.as<int>());
+    }
+    QP_LAUNCH_CHECK();
+    rc = rebin_tail<R>(sp);
+    if (rc) return rc;
+    return check ? check_err(sp, "pmove") : QP_OK;
+}
+
+int species_qdp_into(qp_species *sp, qp_field2d *q)
+{
+    return qp_species_qdp(sp, q);
+}
+
+}  // namespace qp
+
+using namespace qp;
+
+#define SPCHECK(sp) QP_REQUIRE((sp) && (sp)->ctx, QP_ERR_ARG, "%s: null species", __func__)
+#define DISPATCH(c, fn, ...) ((c)->prec == QP_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+int qp_species_create(qp_ctx *c, const qp_species_params *p, double dt, double s,
+                      qp_species **out)
+{
+    QP_REQUIRE(c && p && out, QP_ERR_ARG, "qp_species_create: null argument");
+    QP_REQUIRE(p->ppc[0] >= 1 && p->ppc[1] >= 1 && p->q != 0.0 && p->m != 0.0, QP_ERR_ARG,
+               "qp_species_create: bad species parameters");
+    (void)s;
+    qp_species *sp = new qp_species();
+    sp->ctx = c;
+    sp->p = *p;
+    sp->qbm = p->q / p->m;
+    sp->dt = dt;
+    const int64_t np = (int64_t)(c->nx - 2) * (c->ny - 1) * p->ppc[0] * p->ppc[1];
+    int rc = DISPATCH(c, alloc_particles, sp, np);
+    if (!rc) rc = qp_field2d_create(c, 1, 0, &sp->q);
+    if (!rc) rc = qp_field2d_create(c, 1, 0, &sp->qn);
+    if (!rc) rc = qp_field2d_create(c, 3, 0, &sp->cu);
+    if (!rc) rc = qp_field2d_create(c, 2, 0, &sp->dcu);
+    if (!rc) rc = qp_field2d_create(c, 3, 0, &sp->amu);
+    if (!rc) rc = DISPATCH(c, renew_t, sp);
+    if (rc) { qp_species_destroy(sp); return rc; }
+    *out = sp;
+    return QP_OK;
+}
+
+int qp_species_destroy(qp_species *sp)
+{
+    if (!sp) return QP_OK;
+    if (sp->ctx) (void)hipStreamSynchronize(sp->ctx->stream);
+    for (int k = 0; k < 8; ++k) { sp->a[k].release(); sp->b[k].release(); }
+    sp->key.release(); sp->slot.release(); sp->count.release(); sp->off.release();
+    sp->scan_tmp.release(); sp->err.release();
+    qp_field2d_destroy(sp->q); qp_field2d_destroy(sp->qn); qp_field2d_destroy(sp->cu);
+    qp_field2d_destroy(sp->dcu); qp_field2d_destroy(sp->amu);
+    delete sp;
+    return QP_OK;
+}
+
+int qp_species_renew(qp_species *sp, double s)
+{
+    SPCHECK(sp);
+    (void)s;
+    return DISPATCH(sp->ctx, renew_t, sp);
+}
+
+// qdp_species2d (species2d_class.f03:166-185): this%q = folded deposit;
+// q = this%q + q; q = qn + q
+int qp_species_qdp(qp_species *sp, qp_field2d *q)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(q && q->dim == 1 && q->state == 0, QP_ERR_STATE, "qdp: q must be real space, dim 1");
+    qp_ctx *c = sp->ctx;
+    int rc = c->prec == QP_FP64 ? deposit_q_t<double>(sp, sp->q->rs.as<double>())
+                                : deposit_q_t<float>(sp, sp->q->rs.as<float>());
+    if (rc) return rc;
+    sp->q->state = 0;
+    sp->q->gcells = 1;
+    rc = qp_field2d_add(q, sp->q, q, 0, nullptr, nullptr, nullptr);
+    if (!rc) rc = qp_field2d_add(q, sp->qn, q, 0, nullptr, nullptr, nullptr);
+    return rc;
+}
+
+int qp_species_extpsi(qp_species *sp, const qp_field2d *psi, double dex)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(psi && psi->state == 0, QP_ERR_STATE, "extpsi: psi must be real space");
+    return DISPATCH(sp->ctx, extpsi_t, sp, psi, dex);
+}
+
+// amjdp_species2d (species2d_class.f03:187-218)
+int qp_species_amjdp(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                     const qp_field2d *psit, qp_field2d *cu, qp_field2d *amu, qp_field2d *dcu,
+                     double dex)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(ef && bf && psit && cu && amu && dcu, QP_ERR_ARG, "amjdp: null field");
+    QP_REQUIRE(ef->state == 0 && bf->state == 0 && psit->state == 0, QP_ERR_STATE,
+               "amjdp: fields must be real space");
+    QP_REQUIRE(ef->dim >= 2 && bf->dim == 3 && cu->dim == 3 && dcu->dim >= 2 && amu->dim == 3,
+               QP_ERR_ARG, "amjdp: component counts");
+    int rc = DISPATCH(sp->ctx, amjdp_t, sp, ef, bf, psit, dex);
+    if (!rc) rc = qp_field2d_add(cu, sp->cu, cu, 0, nullptr, nullptr, nullptr);
+    if (!rc) rc = qp_field2d_add(dcu, sp->dcu, dcu, 0, nullptr, nullptr, nullptr);
+    if (!rc) rc = qp_field2d_add(amu, sp->amu, amu, 0, nullptr, nullptr, nullptr);
+    double d = dex;
+    if (!rc) rc = qp_field2d_mult(sp->cu, sp->cu, 0, nullptr, nullptr, &d);
+    return rc;
+}
+
+int qp_species_push(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                    const qp_field2d *psit, double dex)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(ef && bf && psit && ef->state == 0 && bf->state == 0 && psit->state == 0,
+               QP_ERR_STATE, "push: fields must be real space");
+    return DISPATCH(sp->ctx, push_t, sp, ef, bf, psit, dex, true);
+}
+
+// cbq_species2d (species2d_class.f03:399-418)
+int qp_species_cbq(qp_species *sp, qp_field3d *q3, int pos)
+{
+    SPCHECK(sp);
+    const int d1[1] = {1}, d3[1] = {3};
+    int rc = qp_field2d_add(sp->q, sp->q, sp->cu, 1, d1, d1, d3);
+    if (!rc) rc = qp_field2d_fftrk(sp->q, 1);
+    if (!rc) rc = qp_field2d_smooth(sp->q, sp->q);
+    if (!rc) rc = qp_field2d_fftkr(sp->q, 1);
+    if (!rc && q3) rc = qp_field2d_cb(sp->q, q3, pos, 1, d1, d1);
+    return rc;
+}
+
+int qp_species_count(const qp_species *sp, int64_t *np)
+{
+    SPCHECK(sp);
+    *np = sp->np;
+    return QP_OK;
+}
+
+}  // extern "C"
+
+template <typename R>
+static int download_t(const qp_species *sp, double *part, int64_t ld)
+{
+    qp_ctx *c = sp->ctx;
+    std::vector<R> h((size_t)sp->np);
+    for (int k = 0; k < 8; ++k) {
+        QP_HIP(hipMemcpyAsync(h.data(), sp->a[k].p, h.size() * sizeof(R), hipMemcpyDeviceToHost,
+                              c->stream));
+        QP_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < sp->np; ++i) part[k + 8 * i] = (double)h[i];
+    }
+    (void)ld;
+    return QP_OK;
+}
+
+extern "C" {
+
+int qp_species_download(const qp_species *sp, double *part, int64_t ld)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(part && ld >= sp->np, QP_ERR_ARG, "qp_species_download: bad buffer");
+    return sp->ctx->prec == QP_FP64 ? download_t<double>(sp, part, ld)
+                                    : download_t<float>(sp, part, ld);
+}
+
+}  // extern "C"
+
+template <typename R>
+static int upload_t(qp_species *sp, const double *part, int64_t np)
+{
+    qp_ctx *c = sp->ctx;
+    std::vector<R> h((size_t)np);
+    for (int k = 0; k < 8; ++k) {
+        for (int64_t i = 0; i < np; ++i) h[i] = (R)part[k + 8 * i];
+        QP_HIP(hipMemcpyAsync(sp->a[k].p, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice,
+                              c->stream));
+        QP_HIP(hipStreamSynchronize(c->stream));
+    }
+    sp->np = np;
+    return rebin_unsorted<R>(sp);
+}
+
+extern "C" {
+
+int qp_species_upload(qp_species *sp, const double *part, int64_t np, int64_t ld)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(part && np >= 0 && ld >= np, QP_ERR_ARG, "qp_species_upload: bad buffer");
+    QP_REQUIRE((size_t)np * sp->ctx->esize <= sp->a[0].bytes, QP_ERR_ARG,
+               "qp_species_upload: %lld particles exceed the capacity", (long long)np);
+    return sp->ctx->prec == QP_FP64 ? upload_t<double>(sp, part, np)
+                                    : upload_t<float>(sp, part, np);
+}
+
+static int host_cell_counts(const qp_species *sp, std::vector<int> &off)
+{
+    qp_ctx *c = sp->ctx;
+    off.resize((size_t)c->ncells + 1);
+    QP_HIP(hipMemcpyAsync(off.data(), sp->off.p, off.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    QP_HIP(hipStreamSynchronize(c->stream));
+    return QP_OK;
+}
+
+int qp_species_cell_counts(const qp_species *sp, int32_t *counts)
+{
+    SPCHECK(sp);
+    std::vector<int> off;
+    int rc = host_cell_counts(sp, off);
+    if (rc) return rc;
+    qp_ctx *c = sp->ctx;
+    for (int cy = 0; cy < c->ny; ++cy)
+        for (int cx = 0; cx < c->nx; ++cx) {
+            int k = ((cy / TILE) * c->mx1 + cx / TILE) * TCELLS + (cy % TILE) * TILE + cx % TILE;
+            counts[(size_t)cy * c->nx + cx] = off[k + 1] - off[k];
+        }
+    return QP_OK;
+}
+
+int qp_species_tile_counts(const qp_species *sp, int32_t *kpic)
+{
+    SPCHECK(sp);
+    std::vector<int> off;
+    int rc = host_cell_counts(sp, off);
+    if (rc) return rc;
+    for (int t = 0; t < sp->ctx->ntiles; ++t) kpic[t] = off[(t + 1) * TCELLS] - off[t * TCELLS];
+    return QP_OK;
+}
+
+}  // extern "C"
+
+// entry points used by the slice driver without the per-call error sync
+namespace qp {
+int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                         const qp_field2d *psit, double dex)
+{
+    return DISPATCH(sp->ctx, push_t, sp, ef, bf, psit, dex, false);
+}
+int species_check(qp_species *sp) { return check_err(sp, "pmove"); }
+}  // namespace qp

@@ -1,0 +1,228 @@
+// The 3-D step and its xi-slice loop on one GPU (simulation_class.f03:611-765).
+//
+// Field roles and states follow init_sim_fields (simulation_class.f03:267-287):
+// qb, qe, psit, psi, div_vpot (1 comp), fxy, dcu, epw, epwb (2), cu, amu,
+// bxyz (3); cu and epw start in k space.  Every call below enqueues on the
+// context stream; the only host synchronisations are the corrector's stopping
+// test (when a tolerance is given) and the particle check at the slice end.
+#include <cmath>
+#include <vector>
+
+#include "qp_common.h"
+
+namespace qp {
+int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                         const qp_field2d *psit, double dex);
+int species_check(qp_species *sp);
+}  // namespace qp
+
+struct qp_sim {
+    qp_ctx *ctx;
+    std::vector<qp_species *> sp;
+    std::vector<qp_beam *> bm;
+    std::vector<qp_field3d *> q3s;   // optional cbq targets, one per species
+    int iter;
+    qp_field2d *f[QP_F_COUNT];
+    qp_field3d *bexyz, *bbxyz;
+};
+
+using namespace qp;
+
+#define RC(x) do { int _r = (x); if (_r) return _r; } while (0)
+
+extern "C" {
+
+int qp_sim_create(qp_ctx *c, qp_species *const *sp, int nsp, qp_beam *const *bm, int nb, int iter,
+                  qp_sim **out)
+{
+    QP_REQUIRE(c && out && nsp >= 0 && nb >= 0 && iter >= 1, QP_ERR_ARG,
+               "qp_sim_create: bad argument");
+    qp_sim *s = new qp_sim();
+    s->ctx = c;
+    for (int i = 0; i < nsp; ++i) s->sp.push_back(sp[i]);
+    for (int i = 0; i < nb; ++i) s->bm.push_back(bm[i]);
+    s->q3s.assign(nsp, nullptr);
+    s->iter = iter;
+    static const int dims[QP_F_COUNT] = {1, 1, 1, 1, 1, 2, 3, 3, 2, 3, 2, 2};
+    static const int st[QP_F_COUNT] = {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0};
+    int rc = QP_OK;
+    for (int i = 0; i < QP_F_COUNT && !rc; ++i) rc = qp_field2d_create(c, dims[i], st[i], &s->f[i]);
+    if (!rc) rc = qp_field3d_create(c, 3, &s->bexyz);
+    if (!rc) rc = qp_field3d_create(c, 3, &s->bbxyz);
+    if (rc) { qp_sim_destroy(s); return rc; }
+    *out = s;
+    return QP_OK;
+}
+
+int qp_sim_destroy(qp_sim *s)
+{
+    if (!s) return QP_OK;
+    for (int i = 0; i < QP_F_COUNT; ++i) qp_field2d_destroy(s->f[i]);
+    qp_field3d_destroy(s->bexyz);
+    qp_field3d_destroy(s->bbxyz);
+    delete s;
+    return QP_OK;
+}
+
+int qp_sim_field2d(qp_sim *s, int which, qp_field2d **out)
+{
+    QP_REQUIRE(s && out && which >= 0 && which < QP_F_COUNT, QP_ERR_ARG, "qp_sim_field2d: bad argument");
+    *out = s->f[which];
+    return QP_OK;
+}
+
+int qp_sim_field3d(qp_sim *s, qp_field3d **bexyz, qp_field3d **bbxyz)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_field3d: null sim");
+    if (bexyz) *bexyz = s->bexyz;
+    if (bbxyz) *bbxyz = s->bbxyz;
+    return QP_OK;
+}
+
+int qp_sim_species_q3(qp_sim *s, int species, qp_field3d *q3)
+{
+    QP_REQUIRE(s && species >= 0 && species < (int)s->sp.size(), QP_ERR_ARG,
+               "qp_sim_species_q3: bad species index");
+    s->q3s[species] = q3;
+    return QP_OK;
+}
+
+int qp_sim_begin_step(qp_sim *s)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_begin_step: null sim");
+    qp_ctx *c = s->ctx;
+    for (qp_beam *b : s->bm) RC(qp_beam_qdp(b));
+    // pipeline receive on the first stage zeroes these (ufield2d_class.f03:528-531)
+    qp_field2d *cu = s->f[QP_F_CU], *epw = s->f[QP_F_EPW];
+    RC(launch_fill(c, cu->ks.p, cu->dim * c->kplane, 0.0));
+    cu->state = 1;
+    RC(launch_fill(c, epw->ks.p, epw->dim * c->kplane, 0.0));
+    epw->state = 1;
+    RC(qp_field2d_as(s->f[QP_F_FXY], 0.0));
+    RC(qp_field2d_as(s->f[QP_F_PSIT], 0.0));
+    RC(qp_field2d_as(s->f[QP_F_BXYZ], 0.0));
+    // slice-1 copies (simulation_class.f03:639-641)
+    const int d12[2] = {1, 2}, d1[1] = {1}, d3[1] = {3}, d123[3] = {1, 2, 3};
+    RC(qp_field2d_cb(s->f[QP_F_FXY], s->bexyz, 1, 2, d12, d12));
+    RC(qp_field2d_cb(s->f[QP_F_PSIT], s->bexyz, 1, 1, d1, d3));
+    RC(qp_field2d_cb(s->f[QP_F_BXYZ], s->bbxyz, 1, 3, d123, d123));
+    return QP_OK;
+}
+
+// bxyz(1:2) = (-dex*epw(2), dex*epw(1)); bxyz(3) *= dex; bxyz(1) -= epwb(2);
+// bxyz(2) += epwb(1); all in k space (simulation_class.f03:677-680)
+static int assemble_b(qp_sim *s, double dex)
+{
+    qp_field2d *bxyz = s->f[QP_F_BXYZ], *epw = s->f[QP_F_EPW], *epwb = s->f[QP_F_EPWB];
+    const int d12[2] = {1, 2}, d21[2] = {2, 1}, d1[1] = {1}, d2[1] = {2}, d3[1] = {3};
+    const double v1[2] = {-dex, dex}, v3[1] = {dex};
+    RC(qp_field2d_mult(bxyz, epw, 2, d12, d21, v1));
+    RC(qp_field2d_mult(bxyz, bxyz, 1, d3, d3, v3));
+    RC(qp_field2d_sub(bxyz, bxyz, epwb, 1, d1, d1, d2));
+    RC(qp_field2d_add(bxyz, bxyz, epwb, 1, d2, d2, d1));
+    return QP_OK;
+}
+
+// cu -> div -> pot -> psit * (-dex) -> real; cu -> bxyz (k)  (:705-709)
+static int psit_and_b(qp_sim *s, double dex)
+{
+    qp_field2d *cu = s->f[QP_F_CU], *dv = s->f[QP_F_DIVVPOT], *psit = s->f[QP_F_PSIT];
+    const double m = -dex;
+    RC(qp_field2d_div(cu, dv));
+    RC(qp_field2d_pot(dv, psit));
+    RC(qp_field2d_mult(psit, psit, 0, nullptr, nullptr, &m));
+    RC(qp_field2d_fftkr(psit, 1));
+    RC(qp_field2d_bf(cu, s->f[QP_F_BXYZ]));
+    return QP_OK;
+}
+
+int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, double *residual)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_slice_run: null sim");
+    qp_ctx *c = s->ctx;
+    QP_REQUIRE(j >= 1 && j <= c->nz, QP_ERR_ARG, "qp_slice_run: slice %d outside 1..%d", j, c->nz);
+    QP_REQUIRE(iter_max >= 1, QP_ERR_ARG, "qp_slice_run: iter_max %d", iter_max);
+    const double dex = c->dex, dxi = c->dxi;
+    qp_field2d **F = s->f;
+    qp_field2d *qb = F[QP_F_QB], *qe = F[QP_F_QE], *psi = F[QP_F_PSI], *fxy = F[QP_F_FXY];
+    qp_field2d *cu = F[QP_F_CU], *dcu = F[QP_F_DCU], *amu = F[QP_F_AMU], *bxyz = F[QP_F_BXYZ];
+    qp_field2d *psit = F[QP_F_PSIT], *epw = F[QP_F_EPW], *epwb = F[QP_F_EPWB];
+    // beam charge and its transverse field (:650-656)
+    RC(qp_field2d_as(qb, 0.0));
+    for (qp_beam *b : s->bm) RC(qp_beam_qdpcopy(b, qb, j + 1));
+    RC(qp_field2d_fftrk(qb, 1));
+    RC(qp_field2d_elf(qb, epwb));
+    // plasma charge, psi and E_perp (:657-670)
+    RC(qp_field2d_as(qe, 0.0));
+    for (qp_species *sp : s->sp) RC(qp_species_qdp(sp, qe));
+    RC(qp_field2d_fftrk(qe, 1));
+    RC(qp_field2d_pot(qe, psi));
+    RC(qp_field2d_grad(psi, fxy));
+    RC(qp_field2d_fftkr(fxy, 1));
+    RC(qp_field2d_fftkr(psi, 1));
+    for (qp_species *sp : s->sp) RC(qp_species_extpsi(sp, psi, dex));
+    const double dd = dex * dex;
+    RC(qp_field2d_mult(psi, psi, 0, nullptr, nullptr, &dd));
+    // predictor fields from the carried current (:671-675)
+    RC(psit_and_b(s, dex));
+    // corrector passes (:676-710)
+    int l = 0;
+    double res = 0.0;
+    const bool adaptive = tol > 0.0;
+    for (l = 1; l <= iter_max; ++l) {
+        RC(assemble_b(s, dex));
+        RC(qp_field2d_fftkr(bxyz, 2));
+        RC(qp_field2d_as(cu, 0.0));
+        RC(qp_field2d_as(dcu, 0.0));
+        RC(qp_field2d_as(amu, 0.0));
+        for (qp_species *sp : s->sp) RC(qp_species_amjdp(sp, fxy, bxyz, psit, cu, amu, dcu, dex));
+        RC(qp_field2d_mult(cu, cu, 0, nullptr, nullptr, &dex));
+        RC(qp_field2d_mult(amu, amu, 0, nullptr, nullptr, &dex));
+        RC(qp_field2d_mult(dcu, dcu, 0, nullptr, nullptr, &dex));
+        RC(qp_field2d_fftrk(cu, 1));
+        RC(qp_field2d_fftrk(dcu, 1));
+        RC(qp_field2d_fftrk(amu, 3));
+        RC(bfqp_impl(cu, dcu, amu, epw, dd, dex, adaptive ? &res : nullptr));
+        RC(psit_and_b(s, dex));
+        if (adaptive && res < tol) break;
+    }
+    if (l > iter_max) l = iter_max;
+    // species charge diagnostic of the last pass (cbq, :692-695): it reads
+    // only the species' own q and cu, which the later steps leave untouched
+    for (size_t i = 0; i < s->sp.size(); ++i) RC(qp_species_cbq(s->sp[i], s->q3s[i], j + 1));
+    // final B, E_perp, predictor current (:717-726)
+    RC(assemble_b(s, dex));
+    RC(qp_field2d_fftkr(bxyz, 2));
+    const double m1 = -1.0;
+    const int d1[1] = {1}, d2[1] = {2}, d12[2] = {1, 2}, d3[1] = {3}, d123[3] = {1, 2, 3};
+    RC(qp_field2d_mult(fxy, fxy, 0, nullptr, nullptr, &m1));
+    RC(qp_field2d_add(fxy, fxy, bxyz, 1, d1, d1, d2));
+    RC(qp_field2d_sub(fxy, fxy, bxyz, 1, d2, d2, d1));
+    RC(qp_field2d_mult(dcu, dcu, 0, nullptr, nullptr, &dxi));
+    RC(qp_field2d_sub(cu, cu, dcu, 2, d12, d12, d12));
+    // plasma push (:727-730)
+    for (qp_species *sp : s->sp) RC(species_push_nocheck(sp, fxy, bxyz, psit, dex));
+    // slice j+1 of the 3-D wake (:731-735)
+    RC(qp_field2d_mult(fxy, fxy, 0, nullptr, nullptr, &dex));
+    RC(qp_field2d_cb(fxy, s->bexyz, j + 1, 2, d12, d12));
+    RC(qp_field2d_cb(psit, s->bexyz, j + 1, 1, d1, d3));
+    const double vd[2] = {dex, dex};
+    RC(qp_field2d_mult(bxyz, bxyz, 2, d12, d12, vd));
+    RC(qp_field2d_cb(bxyz, s->bbxyz, j + 1, 3, d123, d123));
+    for (qp_species *sp : s->sp) RC(species_check(sp));
+    if (iters_used) *iters_used = l;
+    if (residual) *residual = adaptive ? res : -1.0;
+    return QP_OK;
+}
+
+int qp_sim_end_step(qp_sim *s, double snew)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_end_step: null sim");
+    qp_ctx *c = s->ctx;
+    for (qp_beam *b : s->bm) RC(qp_beam_push(b, s->bexyz, s->bbxyz, c->dex, c->dxi));
+    for (qp_species *sp : s->sp) RC(qp_species_renew(sp, snew));
+    QP_HIP(hipStreamSynchronize(c->stream));
+    return QP_OK;
+}
+
+}  // extern "C"

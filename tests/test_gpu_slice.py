@@ -1,0 +1,123 @@
+"""GPU parity of the composed xi-slice (simulation_class.f03:643-736) and of a
+short 3-D step, against the oracle's restatement of the same sequence
+(oracle/qpo.py Simulation) on the same seeded beam and plasma.
+
+FP64: fields and particle records within 1e-10 relative after each of the
+first slices; per-cell particle counts bit-exact.  FP32: the same quantities
+within 1e-4 relative of the FP64 oracle after a few slices.
+"""
+import numpy as np
+import pytest
+
+from oracle import qpo
+
+qps = pytest.importorskip("qpslice")
+
+pytestmark = pytest.mark.gpu
+
+IND, INDZ = 6, 6          # 64 x 64 x 64
+BOX = (-10.0, 10.0, -10.0, 10.0, 0.0, 10.0)
+
+
+def rel(a, b):
+    s = max(float(np.abs(b).max()), 1e-300)
+    return float(np.abs(np.asarray(a) - b).max()) / s
+
+
+def banner(name):
+    print(f"[gpu-test] {name}", flush=True)
+
+
+def make_pair(precision, peak=4.0, sigma=(0.8, 0.8, 0.5), np_=(32, 32, 64), iter_=2):
+    grid = qpo.Grid(IND, IND, INDZ)
+    bspec = qpo.BeamSpec(center=(0.0, 0.0, 1.5), sigma=sigma, np_=np_, peak=peak)
+    sspec = qpo.SpeciesSpec(ppc=(2, 2))
+    osim = qpo.Simulation(grid, [sspec], [bspec], dt=10.0, iter_=iter_)
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=precision)
+    sp = qps.Species(ctx, ppc=(2, 2))
+    bm = qps.Beam(ctx, center=bspec.center, sigma=bspec.sigma, sigma_v=bspec.sigma_v,
+                  np_=bspec.np_, gamma=bspec.gamma, peak=bspec.peak, dt=10.0)
+    gsim = qps.Sim(ctx, [sp], [bm], iter_=iter_)
+    return ctx, osim, gsim, sp
+
+
+def compare_state(osim, gsim, sp, j, bar):
+    F = gsim.field
+    checks = {
+        "psi": (F(qps.F_PSI).download(0), osim.psi),
+        "psit": (F(qps.F_PSIT).download(0), osim.psit),
+        "fxy": (F(qps.F_FXY).download(0), osim.fxy),
+        "bxyz": (F(qps.F_BXYZ).download(0), osim.bxyz),
+        "cu_k": (F(qps.F_CU).download(1), osim.cu_k),
+        "epw_k": (F(qps.F_EPW).download(1), osim.epw_k),
+    }
+    errs = {k: rel(a, b) for k, (a, b) in checks.items()}
+    e3 = gsim.bexyz
+    for comp in range(3):
+        errs[f"bexyz{comp}"] = rel(e3.plane(comp + 1, j + 1), osim.e3[comp, j])
+        errs[f"bbxyz{comp}"] = rel(gsim.bbxyz.plane(comp + 1, j + 1), osim.b3[comp, j])
+    print(f"slice {j}: " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+    bad = {k: v for k, v in errs.items() if not v < bar}
+    assert not bad, (j, bad)
+
+
+def test_slices_fp64():
+    banner("slices fp64")
+    ctx, osim, gsim, sp = make_pair(qps.FP64)
+    osim.begin_step()
+    gsim.begin_step()
+    for j in range(1, 9):
+        osim.slice(j)
+        gsim.slice(j)
+        compare_state(osim, gsim, sp, j, 1e-10)
+        counts = np.bincount(qpo.cell_of(64, 64, osim.species[0].part), minlength=64 * 64)
+        assert np.array_equal(sp.cell_counts(), counts)
+    got = sp.download()
+    want = osim.species[0].part
+    key = lambda a: a[np.lexsort(a[:, :2].T[::-1])]
+    assert rel(key(got), key(want)) < 1e-10
+    ctx.close()
+
+
+def test_slices_fp32_vs_fp64_oracle():
+    banner("slices fp32")
+    ctx, osim, gsim, sp = make_pair(qps.FP32)
+    osim.begin_step()
+    gsim.begin_step()
+    for j in range(1, 5):
+        osim.slice(j)
+        gsim.slice(j)
+    compare_state(osim, gsim, sp, 4, 1e-4)
+    ctx.close()
+
+
+def test_adaptive_iteration_reaches_tolerance():
+    banner("adaptive corrector")
+    ctx, osim, gsim, sp = make_pair(qps.FP64, peak=10.0, sigma=(0.3, 0.3, 0.5))
+    gsim.begin_step()
+    used = []
+    for j in range(1, 21):
+        it, res = gsim.slice(j, iter_max=20, tol=1e-6)
+        used.append(it)
+        assert it <= 20
+        if it < 20:
+            assert 0.0 <= res < 1e-6
+    print("passes per slice:", used, flush=True)
+    ctx.close()
+
+
+def test_full_step_small():
+    """One whole 3-D step (all 64 slices, beam push, renew) in FP64."""
+    banner("full 3-D step fp64")
+    ctx, osim, gsim, sp = make_pair(qps.FP64, np_=(16, 16, 32))
+    osim.step()
+    gsim.step()
+    for comp in range(3):
+        assert rel(gsim.bexyz.download()[comp], osim.e3[comp]) < 1e-9
+        assert rel(gsim.bbxyz.download()[comp], osim.b3[comp]) < 1e-9
+    got = gsim.beams[0].download()
+    want = osim.beams[0][0]
+    assert got.shape == want.shape
+    srt = lambda a: a[np.lexsort(a.T[::-1])]
+    assert rel(srt(got), srt(want)) < 1e-10
+    ctx.close()
