@@ -2,7 +2,7 @@
 // relativistic push with drop-at-boundary.
 //
 // Beam particles stay in FP64 in both precision modes (positions over a long
-// propagation).  The 3-D charge is accumulated in int64 fixed point (2^-36
+// propagation).  The 3-D charge is accumulated in int64 fixed point (2^-44
 // resolution) with 64-bit atomics, so it is independent of the order in which
 // particles land and therefore identical run to run; it is converted to Real
 // when a slice is copied out (qdpcopy).

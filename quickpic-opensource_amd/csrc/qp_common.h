@@ -27,7 +27,7 @@ constexpr int TILE = 16;                 // cells per tile edge (part2d_class.f0
 constexpr int TCELLS = TILE * TILE;      // cells per tile = threads per particle block
 constexpr int HALO = 4 * TILE;           // border nodes of a 17x17 tile block
 constexpr double TWOPI_REF = 6.28318530717959;  // the reference's 2*pi literal
-constexpr double BEAM_FIX = 68719476736.0;      // 2^36: beam charge fixed point
+constexpr double BEAM_FIX = 17592186044416.0;   // 2^44: beam charge fixed point
 
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);

@@ -260,7 +260,7 @@ def test_beam_loader_and_deposit(ctx):
     b.qdp()
     q3 = np.zeros((1, 65, 65, 65))
     qpo.beam_deposit(want, q3)
-    assert rel(b.charge(), q3[0]) < 1e-10       # 2^-36 fixed point
+    assert rel(b.charge(), q3[0]) < 1e-12       # 2^-44 fixed point
     g, qs = b.moments()
     assert abs(qs - want[:, 6].sum()) <= 1e-12 * abs(want[:, 6].sum())
     gam = np.sqrt(1 + (want[:, 3:6] ** 2).sum(1)).sum()
