@@ -34,23 +34,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kind, qps, np_, n, esize, launches_per_slice=None):
-    """Bytes one launch of a kernel class must move (semantic operands, read or
-    written once): particle words times the particle count plus field planes
-    times the node count."""
-    nodes = (n + 1) * (n + 1)
-    table = {
-        qps.PROF_AMJDP: 8 * np_ + 14 * nodes,      # x,y,px,py,p6,p7,q read, p6 written
-        qps.PROF_PUSH: 12 * np_ + 6 * nodes,       # 7 read, 5 written; 6 field planes
-        qps.PROF_DEPOSIT_Q: 3 * np_ + nodes,
-        qps.PROF_EXTPSI: 6 * np_ + nodes,
-        qps.PROF_SORT: 16 * np_,                   # 8 words read and written
-    }
-    if kind not in table:
-        return None
-    return table[kind] * esize
-
-
 def run_gpu(args, precision, beam_part=None):
     import numpy as np
 
@@ -61,7 +44,8 @@ def run_gpu(args, precision, beam_part=None):
     t0 = time.time()
     sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
     bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0), sigma=(0.3, 0.3, 0.5), sigma_v=(1.0, 1.0, 0.0),
-                  np_=tuple(args.beam_np), gamma=2.0e4, peak=args.peak, dt=10.0,
+                  np_=tuple(args.beam_np) if beam_part is None else (0, 0, 0), gamma=2.0e4,
+                  peak=args.peak, dt=10.0,
                   npmax=0 if beam_part is None else beam_part.shape[0] + 2)
     if beam_part is not None:
         bm.upload(beam_part)
@@ -89,29 +73,26 @@ def run_gpu(args, precision, beam_part=None):
     elapsed = time.perf_counter() - t0
     prof = {qps.PROF_NAMES[k]: ctx.profile_get(k) for k in range(len(qps.PROF_NAMES))}
     ctx.profile(False)
-    esize = 4 if precision == qps.FP32 else 8
-    # dominant kernel class by accumulated time
-    dom = max(range(len(qps.PROF_NAMES)), key=lambda k: prof[qps.PROF_NAMES[k]][0])
-    ms, cnt = prof[qps.PROF_NAMES[dom]]
-    alg = algorithmic_bytes(dom, qps, sp.np, n, esize)
-    roof = None
-    if alg is not None and cnt > 0:
-        avg_s = ms / cnt / 1e3
-        ach = alg / avg_s / 1e9
-        roof = {"kernel": qps.PROF_NAMES[dom], "bound": "hbm", "achieved": round(ach, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_ms": round(ms / cnt, 4),
-                "algorithmic_bytes_per_launch": alg}
-    # all kernel classes, for the record
+    # every kernel class: time, launches and algorithmic bytes (operands read
+    # or written once) counted by the library at each launch
     classes = {}
-    for k, name in enumerate(qps.PROF_NAMES):
-        t, c = prof[name]
+    for name, (t, c, b) in prof.items():
         if c:
-            entry = {"ms_total": round(t, 3), "launches": c, "ms_avg": round(t / c, 4)}
-            b = algorithmic_bytes(k, qps, sp.np, n, esize)
-            if b:
-                entry["GBs"] = round(b / (t / c / 1e3) / 1e9, 1)
-            classes[name] = entry
+            classes[name] = {"ms_total": round(t, 3), "launches": c, "ms_avg": round(t / c, 4),
+                             "GBs": round(b / (t / 1e3) / 1e9, 1) if t > 0 else None}
+    # the roofline object names the dominant particle kernel (the path's
+    # bandwidth-bound hot spot) by accumulated time
+    particle = ("amjdp", "push", "extpsi", "deposit_q", "sort")
+    dom = max(particle, key=lambda k: prof[k][0])
+    t, cnt, b = prof[dom]
+    roof = None
+    if cnt:
+        avg_s = t / cnt / 1e3
+        ach = (b / cnt) / avg_s / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(t / cnt, 4), "algorithmic_bytes_per_launch": b / cnt}
+    total_ms = sum(v[0] for v in prof.values())
     result = {
         "slices_per_s": args.steps / elapsed,
         "ms_per_slice": elapsed / args.steps * 1e3,
@@ -120,6 +101,7 @@ def run_gpu(args, precision, beam_part=None):
         "first_slice": j_first,
         "roofline": roof,
         "kernels": classes,
+        "kernel_ms_per_slice": total_ms / args.steps,
         "np_plasma": sp.np,
         "np_beam": bm.np,
     }
@@ -206,6 +188,7 @@ def main():
         "roofline": r32["roofline"],
         "cpu_baseline": cpu,
         "kernels": r32["kernels"],
+        "kernel_ms_per_step": round(r32["kernel_ms_per_slice"], 4),
     }
     if r64 is not None:
         out["fp64_plain"] = {"value": round(r64["slices_per_s"], 3),

@@ -89,7 +89,10 @@ enum { QP_PROF_FFT = 0, QP_PROF_TRANSPOSE, QP_PROF_KSPACE, QP_PROF_ELEMENTWISE,
        QP_PROF_SORT, QP_PROF_BORDER, QP_PROF_BEAM_DEPOSIT, QP_PROF_BEAM_PUSH,
        QP_PROF_COUNT };
 int qp_ctx_profile(qp_ctx *ctx, int enable);   /* enabling also resets */
-int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count);
+/* bytes: the algorithmic bytes of those launches (operands read or written
+ * once: particle records and field planes), the figure a roofline uses */
+int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
+                       double *bytes);
 
 /* ---------------------------------------------------------------------- */
 /* field2d (field2d_class.f03)                                             */
@@ -202,7 +205,8 @@ typedef struct {
 } qp_beam_params;
 
 /* init_beam3d :61-93; seed 0 is the reference's fixed ranorm state,
- * other seeds start independent streams */
+ * other seeds start independent streams.  np = {0,0,0} with npmax > 0 makes
+ * an empty beam of that capacity, to be filled by qp_beam_upload. */
 int qp_beam_create(qp_ctx *ctx, const qp_beam_params *p, double dt,
                    uint64_t seed, qp_beam **out);
 int qp_beam_destroy(qp_beam *b);

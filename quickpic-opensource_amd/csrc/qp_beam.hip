@@ -359,7 +359,9 @@ extern "C" {
 int qp_beam_create(qp_ctx *c, const qp_beam_params *p, double dt, uint64_t seed, qp_beam **out)
 {
     QP_REQUIRE(c && p && out, QP_ERR_ARG, "qp_beam_create: null argument");
-    QP_REQUIRE(p->q != 0.0 && p->m != 0.0 && p->np[0] > 0 && p->np[1] > 0 && p->np[2] > 0,
+    const bool empty = p->np[0] == 0 && p->np[1] == 0 && p->np[2] == 0;
+    QP_REQUIRE(p->q != 0.0 && p->m != 0.0 &&
+                   ((p->np[0] > 0 && p->np[1] > 0 && p->np[2] > 0) || (empty && p->npmax > 0)),
                QP_ERR_ARG, "qp_beam_create: bad beam parameters");
     int64_t want = (int64_t)p->np[0] * p->np[1] * p->np[2];
     int64_t npmax = p->npmax > 0 ? p->npmax : want + 2;
@@ -373,7 +375,7 @@ int qp_beam_create(qp_ctx *c, const qp_beam_params *p, double dt, uint64_t seed,
     std::vector<double> h[7];
     for (auto &v : h) v.resize((size_t)npmax);
     double qm = 0.0;
-    int64_t n = load_beam(c, *p, seed, h, npmax, &qm);
+    int64_t n = empty ? 0 : load_beam(c, *p, seed, h, npmax, &qm);
     int rc = n < 0 ? (set_error("qp_beam_create: npmax %lld too small", (long long)npmax),
                       QP_ERR_ARG)
                    : beam_alloc(b, npmax);
@@ -403,7 +405,7 @@ int qp_beam_qdp(qp_beam *b)
     QP_HIP(hipMemsetAsync(b->q3.p, 0, b->q3.bytes, c->stream));
     if (b->np > 0) {
         {
-        ProfScope _ps(c, QP_PROF_BEAM_DEPOSIT);
+        ProfScope _ps(c, QP_PROF_BEAM_DEPOSIT, 4.0 * 8.0 * (double)b->np);
         hipLaunchKernelGGL(k_beam_deposit, dim3((unsigned)((b->np + 255) / 256)), dim3(256), 0,
                            c->stream, b->np, b->a[0].as<double>(), b->a[1].as<double>(),
                            b->a[2].as<double>(), b->a[6].as<double>(),
@@ -448,6 +450,7 @@ int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf, double 
     const size_t vol = (size_t)(c->nz + 1) * c->rplane;
     dim3 grid((unsigned)((np + 255) / 256));
     const double qtmh = b->qbm * b->dt, dtcx = b->dt / dex, dtcz = b->dt / dez;
+    ProfScope _ps(c, QP_PROF_BEAM_PUSH, 9.0 * 8.0 * (double)np);   // 6 read, 6 written, 3 q/flags
     if (c->prec == QP_FP64)
         hipLaunchKernelGGL(k_beam_push<double>, grid, dim3(256), 0, c->stream, np,
                            b->a[0].as<double>(), b->a[1].as<double>(), b->a[2].as<double>(),
@@ -475,7 +478,7 @@ int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf, double 
     Ptr7 src, dst;
     for (int k = 0; k < 7; ++k) { src.p[k] = b->a[k].as<double>(); dst.p[k] = b->b[k].as<double>(); }
     {
-    ProfScope _ps(c, QP_PROF_BEAM_PUSH);
+    ProfScope _ps(c, QP_PROF_BEAM_PUSH, 7.0 * 8.0 * 2.0 * (double)np);
     hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, c->stream, np, b->alive.as<int>(),
                        b->pos.as<int>(), src, dst);
     }

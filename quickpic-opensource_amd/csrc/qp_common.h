@@ -66,6 +66,7 @@ namespace qp {
 struct ProfEv {
     hipEvent_t a, b;
     int kind;
+    double bytes;
 };
 }  // namespace qp
 
@@ -92,6 +93,7 @@ struct qp_ctx {
     std::vector<hipEvent_t> prof_pool;
     double prof_ms[QP_PROF_COUNT];
     int64_t prof_n[QP_PROF_COUNT];
+    double prof_bytes[QP_PROF_COUNT];
 };
 
 struct qp_field2d {
@@ -126,13 +128,16 @@ int bfqp_impl(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy,
               double c, double dex, double *resid_out);
 
 // ---- kernel-class timing ----
-int prof_begin(qp_ctx *c, int kind);         // returns a slot or -1
+int prof_begin(qp_ctx *c, int kind, double bytes);   // returns a slot or -1
 void prof_end(qp_ctx *c, int slot);
 
+// Brackets one launch with events while profiling is on; bytes is the
+// launch's algorithmic traffic.
 struct ProfScope {
     qp_ctx *c;
     int slot;
-    ProfScope(qp_ctx *cc, int kind) : c(cc), slot(cc->prof_on ? prof_begin(cc, kind) : -1) {}
+    ProfScope(qp_ctx *cc, int kind, double bytes = 0.0)
+        : c(cc), slot(cc->prof_on ? prof_begin(cc, kind, bytes) : -1) {}
     ~ProfScope() { if (slot >= 0) prof_end(c, slot); }
 };
 

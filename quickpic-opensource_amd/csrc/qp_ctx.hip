@@ -125,9 +125,9 @@ static hipEvent_t pool_get(qp_ctx *c)
     return e;
 }
 
-int prof_begin(qp_ctx *c, int kind)
+int prof_begin(qp_ctx *c, int kind, double bytes)
 {
-    ProfEv ev{pool_get(c), pool_get(c), kind};
+    ProfEv ev{pool_get(c), pool_get(c), kind, bytes};
     if (!ev.a || !ev.b) return -1;
     if (hipEventRecord(ev.a, c->stream) != hipSuccess) return -1;
     c->prof_pending.push_back(ev);
@@ -147,6 +147,7 @@ static int prof_resolve(qp_ctx *c)
         if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
             c->prof_ms[e.kind] += ms;
             c->prof_n[e.kind] += 1;
+            c->prof_bytes[e.kind] += e.bytes;
         }
         c->prof_pool.push_back(e.a);
         c->prof_pool.push_back(e.b);
@@ -227,18 +228,23 @@ int qp_ctx_profile(qp_ctx *c, int enable)
     QP_REQUIRE(c, QP_ERR_ARG, "qp_ctx_profile: null context");
     int rc = prof_resolve(c);
     if (rc) return rc;
-    for (int k = 0; k < QP_PROF_COUNT; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    for (int k = 0; k < QP_PROF_COUNT; ++k) {
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+        c->prof_bytes[k] = 0.0;
+    }
     c->prof_on = enable ? 1 : 0;
     return QP_OK;
 }
 
-int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count)
+int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count, double *bytes)
 {
     QP_REQUIRE(c && kind >= 0 && kind < QP_PROF_COUNT, QP_ERR_ARG, "qp_ctx_profile_get: bad argument");
     int rc = prof_resolve(c);
     if (rc) return rc;
     if (total_ms) *total_ms = c->prof_ms[kind];
     if (count) *count = c->prof_n[kind];
+    if (bytes) *bytes = c->prof_bytes[kind];
     return QP_OK;
 }
 

@@ -173,7 +173,7 @@ static int rowfft(qp_ctx *c, const R *src, size_t src_cs, int ld_src, R *dst, si
     for (int i = 0; i < dim; ++i) tmask |= (types[i] & 1) << i;
     dim3 grid((nrows + 1) / 2, dim);
     {
-    ProfScope _ps(c, QP_PROF_FFT);
+    ProfScope _ps(c, QP_PROF_FFT, 2.0 * dim * nrows * (n + 1.0) * sizeof(R));
     hipLaunchKernelGGL(k_rowfft<R>, grid, dim3(256), lds, c->stream, src, src_cs, ld_src, dst,
                        dst_cs, ld_dst, nrows, n, ilog2(M), tmask, (R)scale, W, guard_row);
     }
@@ -187,7 +187,7 @@ static int transpose(qp_ctx *c, const R *src, size_t src_cs, int ld_src, R *dst,
 {
     dim3 grid((cols + 31) / 32, (rows + 31) / 32, dim);
     {
-    ProfScope _ps(c, QP_PROF_TRANSPOSE);
+    ProfScope _ps(c, QP_PROF_TRANSPOSE, 2.0 * dim * (double)rows * cols * sizeof(R));
     hipLaunchKernelGGL(k_transpose<R>, grid, dim3(256), 0, c->stream, src, src_cs, ld_src, dst,
                        dst_cs, ld_dst, rows, cols);
     }

@@ -84,7 +84,7 @@ template <typename R>
 static int fill_t(qp_ctx *c, void *p, size_t n, double v)
 {
     {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE);
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, (double)n * sizeof(R));
     hipLaunchKernelGGL(k_fill<R>, grid_for(n), dim3(256), 0, c->stream, (R *)p, n, (R)v);
     }
     QP_LAUNCH_CHECK();
@@ -100,7 +100,7 @@ template <typename R>
 static int fold_t(qp_ctx *c, void *planes, int dim)
 {
     {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE);
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 3.0 * dim * (c->nx + 1) * sizeof(R));
     hipLaunchKernelGGL(k_guard_fold<R>, dim3((c->nx + 256) / 256, dim), dim3(256), 0, c->stream,
                        (R *)planes, c->rplane, c->ldr, c->nx, c->ny);
     }
@@ -118,7 +118,7 @@ static int copyguard_t(qp_field2d *f)
 {
     qp_ctx *c = f->ctx;
     {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE);
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 2.0 * f->dim * (c->nx + 1) * sizeof(R));
     hipLaunchKernelGGL(k_copyguard<R>, dim3((c->ldr + 255) / 256, f->dim), dim3(256), 0,
                        c->stream, f->rs.as<R>(), c->rplane, c->ldr, c->ny);
     }
@@ -350,7 +350,7 @@ int bfqp_resid(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy
     qp_ctx *c = cu->ctx;
     double *part = c->red.as<double>();
     {
-    ProfScope _ps(c, QP_PROF_KSPACE);
+    ProfScope _ps(c, QP_PROF_KSPACE, 16.0 * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R));
     hipLaunchKernelGGL(k_bfqp<R>, dim3(BFQP_BLOCKS), dim3(256), 0, c->stream, kgrid(c),
                        c->ffg.as<R>(), c->ffs.as<R>(), cu->ks.as<R>(), dcu->ks.as<R>(),
                        amu->ks.as<R>(), bxy->ks.as<R>(), (R)cc, (R)(1.0 / dex), part);
@@ -426,7 +426,7 @@ static int mapop_t(qp_ctx *c, void *dst, const void *a1, const void *a2, size_t 
                    const MapArgs &m)
 {
     {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE);
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, (m.op == 2 ? 2.0 : 3.0) * m.n * (double)cs * sizeof(R));
     hipLaunchKernelGGL(k_mapop<R>, grid_for(cs), dim3(256), 0, c->stream, (R *)dst,
                        (const R *)a1, (const R *)a2, cs, cs, m);
     }
@@ -482,7 +482,7 @@ template <typename R>
 static int slice_copy_t(qp_ctx *c, R *dst, const R *src, int add)
 {
     {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE);
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, (add ? 3.0 : 2.0) * (double)c->rplane * sizeof(R));
     hipLaunchKernelGGL(k_plane_copy<R>, grid_for(c->rplane), dim3(256), 0, c->stream, dst, src,
                        c->rplane, add);
     }
@@ -623,6 +623,8 @@ static int kop_launch(int op, qp_field2d *in, qp_field2d *out)
     KGrid g = kgrid(c);
     const R *a = in->ks.as<R>();
     R *o = out->ks.as<R>();
+    static const double planes[6] = {3, 3, 3, 3, 5, 8};   // read + written + tables
+    ProfScope _ps(c, QP_PROF_KSPACE, planes[op] * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R));
     switch (op) {
     case 0: hipLaunchKernelGGL(k_div<R>, kgrid_blocks(c), dim3(256), 0, c->stream, g, a, o); break;
     case 1: hipLaunchKernelGGL(k_grad<R>, kgrid_blocks(c), dim3(256), 0, c->stream, g, a, o); break;

@@ -465,7 +465,7 @@ static int rebin_tail(qp_species *sp)
     QP_HIP(hipcub::DeviceScan::ExclusiveSum(sp->scan_tmp.p, tb, sp->count.as<int>(),
                                             sp->off.as<int>(), c->ncells + 1, c->stream));
     {
-    ProfScope _ps(c, QP_PROF_SORT);
+    ProfScope _ps(c, QP_PROF_SORT, 16.0 * sizeof(R) * (double)sp->np);
     hipLaunchKernelGGL(k_scatter<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
                        soa<R>(sp->a), soa<R>(sp->b), sp->key.as<int>(), sp->slot.as<int>(),
                        sp->off.as<int>());
@@ -506,7 +506,7 @@ static int rebin_unsorted(qp_species *sp)
     int rc = zero_counts(sp);
     if (rc) return rc;
     {
-    ProfScope _ps(c, QP_PROF_SORT);
+    ProfScope _ps(c, QP_PROF_SORT, 2.0 * sizeof(R) * (double)sp->np);
     hipLaunchKernelGGL(k_keys<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
                        sp->a[0].as<R>(), sp->a[1].as<R>(), c->nx, c->ny, c->mx1,
                        sp->key.as<int>(), sp->slot.as<int>(), sp->count.as<int>(),
@@ -524,7 +524,7 @@ static int deposit_border(qp_ctx *c, const PlaneSet<R> &out, int ncomp, int fold
     TileGeom g = geom(c);
     int n = (c->my1 + 1) * (c->nx + 1) + (c->mx1 + 1) * c->my1 * (TILE - 1);
     {
-    ProfScope _ps(c, QP_PROF_BORDER);
+    ProfScope _ps(c, QP_PROF_BORDER, 2.0 * ncomp * (double)n * sizeof(R));
     hipLaunchKernelGGL(k_deposit_border<R>, dim3((n + 255) / 256, ncomp), dim3(256), 0, c->stream,
                        g, c->halo.as<R>(), (size_t)c->ntiles * HALO, out, fold);
     }
@@ -538,7 +538,7 @@ static int deposit_q_t(qp_species *sp, R *plane)
 {
     qp_ctx *c = sp->ctx;
     {
-    ProfScope _ps(c, QP_PROF_DEPOSIT_Q);
+    ProfScope _ps(c, QP_PROF_DEPOSIT_Q, (3.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(k_deposit_q<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), plane, c->halo.as<R>());
     }
@@ -619,7 +619,7 @@ static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
 {
     qp_ctx *c = sp->ctx;
     {
-    ProfScope _ps(c, QP_PROF_EXTPSI);
+    ProfScope _ps(c, QP_PROF_EXTPSI, (6.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(k_extpsi<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), psi->rs.as<R>(), (R)sp->qbm, (R)dex);
     }
@@ -640,7 +640,7 @@ static int amjdp_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     out.p[5] = amu; out.p[6] = amu + pl; out.p[7] = amu + 2 * pl;
     const size_t hcs = (size_t)c->ntiles * HALO;
     {
-    ProfScope _ps(c, QP_PROF_AMJDP);
+    ProfScope _ps(c, QP_PROF_AMJDP, (8.0 * sp->np + 14.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(k_amjdp<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
                        psit->rs.as<R>(), pl, out, c->halo.as<R>(), hcs, (R)sp->qbm, (R)sp->dt,
@@ -660,7 +660,7 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     int rc = zero_counts(sp);
     if (rc) return rc;
     {
-    ProfScope _ps(c, QP_PROF_PUSH);
+    ProfScope _ps(c, QP_PROF_PUSH, (12.0 * sp->np + 6.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(k_push<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
                        psit->rs.as<R>(), c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex,

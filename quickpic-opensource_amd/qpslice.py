@@ -121,10 +121,11 @@ class Ctx:
         _call(lib().qp_ctx_profile, self.h, 1 if enable else 0)
 
     def profile_get(self, kind):
-        """(total milliseconds, launches) of one kernel class (PROF_*)."""
-        ms, n = C.c_double(), C.c_int64()
-        _call(lib().qp_ctx_profile_get, self.h, kind, C.byref(ms), C.byref(n))
-        return ms.value, n.value
+        """(total milliseconds, launches, algorithmic bytes) of one kernel
+        class (PROF_*)."""
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        _call(lib().qp_ctx_profile_get, self.h, kind, C.byref(ms), C.byref(n), C.byref(b))
+        return ms.value, n.value, b.value
 
     def close(self):
         """Destroys the context after every object created on it."""
