@@ -90,34 +90,6 @@ __device__ __forceinline__ int halo_index(int i, int j)
     return 49 + (j - 1);
 }
 
-// Combine this thread's cell-corner sums of one component into the tile's
-// nodes: interior nodes straight to the output plane, border nodes to halo.
-template <typename R>
-__device__ __forceinline__ void emit_tile(R c00, R c10, R c01, R c11, R *lds, R *plane, int ld,
-                                          R *halo, int ntiles)
-{
-    const int tid = threadIdx.x, t = blockIdx.x;
-    lds[tid] = c00;
-    lds[TCELLS + tid] = c10;
-    lds[2 * TCELLS + tid] = c01;
-    lds[3 * TCELLS + tid] = c11;
-    __syncthreads();
-    (void)ntiles;
-    for (int n = tid; n < 17 * 17; n += TCELLS) {
-        const int i = n % 17, j = n / 17;
-        R s = R(0);
-        if (i < TILE && j < TILE) s += lds[j * TILE + i];
-        if (i > 0 && j < TILE) s += lds[TCELLS + j * TILE + i - 1];
-        if (i < TILE && j > 0) s += lds[2 * TCELLS + (j - 1) * TILE + i];
-        if (i > 0 && j > 0) s += lds[3 * TCELLS + (j - 1) * TILE + i - 1];
-        if (i >= 1 && i < TILE && j >= 1 && j < TILE)
-            plane[(size_t)(j) * ld + i] = s;
-        else
-            halo[(size_t)t * HALO + halo_index(i, j)] = s;
-    }
-    __syncthreads();
-}
-
 template <typename R> struct PlaneSet {
     R *p[8];
 };
@@ -168,179 +140,285 @@ __global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t 
     out.p[comp][(size_t)Y * g.ldr + X] = v;
 }
 
-// PPGPPOST2L (part2d_lib77.f:194-309): q(n,m) += qm (1-dx)(1-dy) etc.
+// LDS slot of staged particle p: one pad word every 32 entries spreads the
+// per-cell strided reads over the banks
+__device__ __forceinline__ int lpad(int p) { return p + (p >> 5); }
+
+template <int CH> struct Stage {
+    static constexpr int N = CH + CH / 32 + 1;
+};
+
+// Interior nodes of the tile's 17x17 block to the output plane, border nodes
+// to the halo buffer (summed over tiles by k_deposit_border).
 template <typename R>
+__device__ __forceinline__ void emit_nodes(const R *nacc, R *plane, int ldr, R *halo,
+                                           int tile, int tx, int ty)
+{
+    for (int n = threadIdx.x; n < 17 * 17; n += TCELLS) {
+        const int i = n % 17, j = n / 17;
+        const R v = nacc[n];
+        if (i >= 1 && i < TILE && j >= 1 && j < TILE)
+            plane[(size_t)(ty * TILE + j) * ldr + tx * TILE + i] = v;
+        else
+            halo[(size_t)tile * HALO + halo_index(i, j)] = v;
+    }
+}
+
+// Deposit skeleton.  The tile's cells are taken RG rows at a time; inside a
+// group every cell owns T = 16/RG consecutive lanes (lane cl*T + sub), lane
+// sub taking particles sub, sub+T, ... of its cell.  The group's particles
+// (a contiguous run of the sorted arrays) are staged through LDS in chunks of
+// at most CH with coalesced loads.  Per-lane register sums are reduced over
+// the T lanes with xor shuffles and added to the tile's node accumulators in
+// four phases (one per cell corner), so every sum has a fixed order and no
+// atomic touches the deposit.
+
+// PPGPPOST2L (part2d_lib77.f:194-309): q(n,m) += qm (1-dx)(1-dy) etc.
+template <typename R, int RG, int CH>
 __global__ void __launch_bounds__(256)
 k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, R *plane, R *halo)
 {
-    __shared__ R lds[4 * TCELLS];
-    int cx, cy;
-    my_cell(g, cx, cy);
-    const int key = blockIdx.x * TCELLS + threadIdx.x;
-    const int i0 = off[key], i1 = off[key + 1];
-    const R fx = (R)cx, fy = (R)cy;
-    R a00 = 0, a10 = 0, a01 = 0, a11 = 0;
-    for (int i = i0; i < i1; ++i) {
-        R x = s.x[i], y = s.y[i], qm = s.q[i];
-        R dxp = qm * (x - fx);
-        R dyp = y - fy;
-        R amx = qm - dxp;
-        R amy = R(1) - dyp;
-        a00 += amx * amy;
-        a10 += dxp * amy;
-        a01 += amx * dyp;
-        a11 += dxp * dyp;
+    constexpr int T = TILE / RG, NP = Stage<CH>::N;
+    extern __shared__ unsigned char smem_raw[];
+    R *sx = reinterpret_cast<R *>(smem_raw), *sy = sx + NP, *sq = sy + NP, *nacc = sq + NP;
+    const int tid = threadIdx.x, tile = blockIdx.x;
+    const int tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
+    for (int n = tid; n < 289; n += TCELLS) nacc[n] = R(0);
+    const int cl = tid / T, sub = tid % T;
+    for (int row = 0; row < TILE; row += RG) {
+        const int lrow = row + cl / TILE, lcol = cl % TILE;
+        const int key = kb + lrow * TILE + lcol;
+        const int i0 = off[key], i1 = off[key + 1];
+        const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
+        const R fx = (R)(tx * TILE + lcol), fy = (R)(ty * TILE + lrow);
+        R a00 = 0, a10 = 0, a01 = 0, a11 = 0;
+        for (int c0 = G0; c0 < G1; c0 += CH) {
+            const int n = min(CH, G1 - c0);
+            __syncthreads();
+            for (int p = tid; p < n; p += TCELLS) {
+                const int l = lpad(p);
+                sx[l] = s.x[c0 + p];
+                sy[l] = s.y[c0 + p];
+                sq[l] = s.q[c0 + p];
+            }
+            __syncthreads();
+            int a = i0 + sub;
+            if (a < c0) a += ((c0 - a + T - 1) / T) * T;
+            const int b = min(i1, c0 + n);
+            for (int i = a; i < b; i += T) {
+                const int l = lpad(i - c0);
+                R qm = sq[l];
+                R dxp = qm * (sx[l] - fx);
+                R dyp = sy[l] - fy;
+                R amx = qm - dxp;
+                R amy = R(1) - dyp;
+                a00 += amx * amy;
+                a10 += dxp * amy;
+                a01 += amx * dyp;
+                a11 += dxp * dyp;
+            }
+        }
+#pragma unroll
+        for (int o = T / 2; o > 0; o >>= 1) {
+            a00 += __shfl_xor(a00, o);
+            a10 += __shfl_xor(a10, o);
+            a01 += __shfl_xor(a01, o);
+            a11 += __shfl_xor(a11, o);
+        }
+        const R av[4] = {a00, a10, a01, a11};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            __syncthreads();
+            if (sub == 0) nacc[(lrow + (k >> 1)) * 17 + lcol + (k & 1)] += av[k];
+        }
     }
-    const int tx = blockIdx.x % g.mx1, ty = blockIdx.x / g.mx1;
-    emit_tile<R>(a00, a10, a01, a11, lds, plane + (size_t)(ty * TILE) * g.ldr + tx * TILE, g.ldr,
-                 halo, g.mx1 * g.my1);
+    __syncthreads();
+    emit_nodes(nacc, plane, g.ldr, halo, tile, tx, ty);
 }
 
-// WPGPSIPOST2L_QP (part2d_lib77.f:2207-2284)
+// WPGPSIPOST2L_QP (part2d_lib77.f:2207-2284), one thread per particle
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_extpsi(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *psi, R qbm, R dex)
+k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R dex)
 {
-    int cx, cy;
-    my_cell(g, cx, cy);
-    const int key = blockIdx.x * TCELLS + threadIdx.x;
-    const int i0 = off[key], i1 = off[key + 1];
-    if (i0 == i1) return;
-    const Corners<R> f = load_corners(psi, g.ldr, cx, cy);
-    const R dx2 = dex * dex, fx = (R)cx, fy = (R)cy;
-    for (int i = i0; i < i1; ++i) {
-        R dxp = s.x[i] - fx, dyp = s.y[i] - fy;
-        R amx = R(1) - dxp, amy = R(1) - dyp;
-        R vx = s.px[i], vy = s.py[i];
-        R dx = gather(f, amx, amy, dxp, dyp);
-        dx = -(dx * dx2 * qbm);
-        s.p7[i] = R(1) + dx;
-        s.p6[i] = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
-    }
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const R x = s.x[i], y = s.y[i];
+    const int cx = min(max((int)x, 0), nx - 1), cy = min(max((int)y, 0), ny - 1);
+    const Corners<R> f = load_corners(psi, ldr, cx, cy);
+    const R dx2 = dex * dex;
+    R dxp = x - (R)cx, dyp = y - (R)cy;
+    R amx = R(1) - dxp, amy = R(1) - dyp;
+    R vx = s.px[i], vy = s.py[i];
+    R dx = gather(f, amx, amy, dxp, dyp);
+    dx = -(dx * dx2 * qbm);
+    s.p7[i] = R(1) + dx;
+    s.p6[i] = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
 }
 
 // PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768): half Boris step with the B_z
 // rotation, deposit of J (3), dJ/dxi (2) and momentum flux (3); p6 updated.
-template <typename R>
+template <typename R, int RG, int CH>
 __global__ void __launch_bounds__(256)
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
         const R *psit, size_t cs, PlaneSet<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex)
 {
-    __shared__ R lds[4 * TCELLS];
-    int cx, cy;
-    my_cell(g, cx, cy);
-    const int key = blockIdx.x * TCELLS + threadIdx.x;
-    const int i0 = off[key], i1 = off[key + 1];
-    R acc[8][4];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
-    if (i1 > i0) {
+    constexpr int T = TILE / RG, NP = Stage<CH>::N;
+    extern __shared__ unsigned char smem_raw[];
+    R *sx = reinterpret_cast<R *>(smem_raw), *sy = sx + NP, *spx = sy + NP, *spy = spx + NP;
+    R *sp6 = spy + NP, *sp7 = sp6 + NP, *sq = sp7 + NP, *nacc = sq + NP;   // nacc: 8 x 289
+    const int tid = threadIdx.x, tile = blockIdx.x;
+    const int tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
+    for (int n = tid; n < 8 * 289; n += TCELLS) nacc[n] = R(0);
+    const int cl = tid / T, sub = tid % T;
+    const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
+    for (int row = 0; row < TILE; row += RG) {
+        const int lrow = row + cl / TILE, lcol = cl % TILE;
+        const int key = kb + lrow * TILE + lcol;
+        const int i0 = off[key], i1 = off[key + 1];
+        const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
+        const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
         const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
         const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
         const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
         const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
         const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
         const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
-        const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
         const R fx = (R)cx, fy = (R)cy;
-        for (int i = i0; i < i1; ++i) {
-            R p6 = s.p6[i], p7 = s.p7[i], qm = s.q[i];
-            R dxp = s.x[i] - fx, dyp = s.y[i] - fy;
-            R amx = R(1) - dxp, amy = R(1) - dyp;
-            R inv_part_7 = R(1) / p7;
-            R qtmh1 = qtmh * inv_part_7;
-            R qtmh2 = qtmh1 * p6;
-            R dx = gather(ex, amx, amy, dxp, dyp);
-            R dy = gather(ey, amx, amy, dxp, dyp);
-            R dz = gather(ez, amx, amy, dxp, dyp);
-            R ox = gather(bx, amx, amy, dxp, dyp);
-            R oy = gather(by, amx, amy, dxp, dyp);
-            R oz = gather(bz, amx, amy, dxp, dyp);
-            R ddx = (R(-1)) * qtmh2 * dx + qtmh * oy;
-            R ddy = (R(-1)) * qtmh2 * dy - qtmh * ox;
-            R vx = s.px[i], vy = s.py[i];
-            R acx = vx + ddx, acy = vy + ddy;
-            R omzt = qtmh1 * oz;
-            R omt = omzt * omzt;
-            R anorm = R(2) / (R(1) + omt);
-            R rot1 = R(0.5) * (R(1) - omt);
-            R rot2 = omzt;
-            R v1 = (rot1 * acx + rot2 * acy) * anorm + ddx;
-            R v2 = (rot1 * acy - rot2 * acx) * anorm + ddy;
-            R qm1 = qm * inv_part_7;
-            amx = qm1 * amx;
-            dxp = qm1 * dxp;
-            ox = R(0.5) * (v1 + vx);
-            oy = R(0.5) * (v2 + vy);
-            oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
-            s.p6[i] = p7 + oz;
-            oz = oz * idex;
-            vx = (v1 - vx) * dti;
-            vy = (v2 - vy) * dti;
-            dz = qbm * (dz + (dx * ox + dy * oy) * dex * dex * inv_part_7);
-            R w[4] = {amx * amy, dxp * amy, amx * dyp, dxp * dyp};
-            v1 = ox * ox * inv_part_7;
-            v2 = oy * oy * inv_part_7;
-            R v3 = ox * oy * inv_part_7;
-            vx = vx + ox * dz * inv_part_7;
-            vy = vy + oy * dz * inv_part_7;
+        R acc[8][4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                acc[0][k] += ox * w[k];
-                acc[1][k] += oy * w[k];
-                acc[2][k] += oz * w[k];
-                acc[3][k] += vx * w[k];
-                acc[4][k] += vy * w[k];
-                acc[5][k] += v1 * w[k];
-                acc[6][k] += v2 * w[k];
-                acc[7][k] += v3 * w[k];
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
+        for (int c0 = G0; c0 < G1; c0 += CH) {
+            const int n = min(CH, G1 - c0);
+            __syncthreads();
+            for (int p = tid; p < n; p += TCELLS) {
+                const int l = lpad(p), q = c0 + p;
+                sx[l] = s.x[q];
+                sy[l] = s.y[q];
+                spx[l] = s.px[q];
+                spy[l] = s.py[q];
+                sp6[l] = s.p6[q];
+                sp7[l] = s.p7[q];
+                sq[l] = s.q[q];
+            }
+            __syncthreads();
+            int a = i0 + sub;
+            if (a < c0) a += ((c0 - a + T - 1) / T) * T;
+            const int b = min(i1, c0 + n);
+            for (int i = a; i < b; i += T) {
+                const int l = lpad(i - c0);
+                R p6 = sp6[l], p7 = sp7[l], qm = sq[l];
+                R dxp = sx[l] - fx, dyp = sy[l] - fy;
+                R amx = R(1) - dxp, amy = R(1) - dyp;
+                R inv_part_7 = R(1) / p7;
+                R qtmh1 = qtmh * inv_part_7;
+                R qtmh2 = qtmh1 * p6;
+                R dx = gather(ex, amx, amy, dxp, dyp);
+                R dy = gather(ey, amx, amy, dxp, dyp);
+                R dz = gather(ez, amx, amy, dxp, dyp);
+                R ox = gather(bx, amx, amy, dxp, dyp);
+                R oy = gather(by, amx, amy, dxp, dyp);
+                R oz = gather(bz, amx, amy, dxp, dyp);
+                R ddx = (R(-1)) * qtmh2 * dx + qtmh * oy;
+                R ddy = (R(-1)) * qtmh2 * dy - qtmh * ox;
+                R vx = spx[l], vy = spy[l];
+                R acx = vx + ddx, acy = vy + ddy;
+                R omzt = qtmh1 * oz;
+                R omt = omzt * omzt;
+                R anorm = R(2) / (R(1) + omt);
+                R rot1 = R(0.5) * (R(1) - omt);
+                R rot2 = omzt;
+                R v1 = (rot1 * acx + rot2 * acy) * anorm + ddx;
+                R v2 = (rot1 * acy - rot2 * acx) * anorm + ddy;
+                R qm1 = qm * inv_part_7;
+                amx = qm1 * amx;
+                dxp = qm1 * dxp;
+                ox = R(0.5) * (v1 + vx);
+                oy = R(0.5) * (v2 + vy);
+                oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
+                sp6[l] = p7 + oz;
+                oz = oz * idex;
+                vx = (v1 - vx) * dti;
+                vy = (v2 - vy) * dti;
+                dz = qbm * (dz + (dx * ox + dy * oy) * dex * dex * inv_part_7);
+                const R w[4] = {amx * amy, dxp * amy, amx * dyp, dxp * dyp};
+                v1 = ox * ox * inv_part_7;
+                v2 = oy * oy * inv_part_7;
+                R v3 = ox * oy * inv_part_7;
+                vx = vx + ox * dz * inv_part_7;
+                vy = vy + oy * dz * inv_part_7;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    acc[0][k] += ox * w[k];
+                    acc[1][k] += oy * w[k];
+                    acc[2][k] += oz * w[k];
+                    acc[3][k] += vx * w[k];
+                    acc[4][k] += vy * w[k];
+                    acc[5][k] += v1 * w[k];
+                    acc[6][k] += v2 * w[k];
+                    acc[7][k] += v3 * w[k];
+                }
+            }
+            __syncthreads();
+            for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
+        }
+#pragma unroll
+        for (int o = T / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[c][k] += __shfl_xor(acc[c][k], o);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            __syncthreads();
+            if (sub == 0) {
+                const int node = (lrow + (k >> 1)) * 17 + lcol + (k & 1);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) nacc[c * 289 + node] += acc[c][k];
             }
         }
     }
-    const int tx = blockIdx.x % g.mx1, ty = blockIdx.x / g.mx1;
-    const size_t base = (size_t)(ty * TILE) * g.ldr + tx * TILE;
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-        emit_tile<R>(acc[c][0], acc[c][1], acc[c][2], acc[c][3], lds, out.p[c] + base, g.ldr,
-                     halo + c * halo_cs, g.mx1 * g.my1);
+        emit_nodes(nacc + c * 289, out.p[c], g.ldr, halo + c * halo_cs, tile, tx, ty);
 }
 
 // PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
-// periodic wrap of the reorder (PPPORDER2LB :1936-1977).  Emits every
-// particle's new cell key and its rank in that cell.  A particle whose new
-// tile is not its old tile plus the leaving direction (the condition
-// PPPCHECK2L reports) sets *err.
+// periodic wrap of the reorder (PPPORDER2LB :1936-1977); one thread per
+// particle.  Emits every particle's new cell key and its rank in that cell;
+// lanes of a wave bound for the same cell take one atomic between them.  A
+// particle whose new tile is not its old tile plus the leaving direction (the
+// condition PPPCHECK2L reports) sets *err.
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
-       const R *psit, size_t cs, R qbm, R dt, R dex, int *newkey, int *slot, int *count, int *err)
+k_push(int64_t np, TileGeom g, Soa<R> s, const R *fxy, const R *bxy, const R *psit, size_t cs,
+       R qbm, R dt, R dex, int *newkey, int *slot, int *count, int *err)
 {
-    int cx, cy;
-    my_cell(g, cx, cy);
-    const int key = blockIdx.x * TCELLS + threadIdx.x;
-    const int i0 = off[key], i1 = off[key + 1];
-    if (i0 == i1) return;
-    const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
-    const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
-    const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
-    const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
-    const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
-    const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
-    const R idex = R(1) / dex, qtmh = R(0.5) * qbm * dt;
-    const R anx = (R)g.nx, any = (R)g.ny;
-    const R fx = (R)cx, fy = (R)cy;
-    const int tx0 = cx >> 4, ty0 = cy >> 4;
-    const R edgelx = (R)(tx0 * TILE), edgely = (R)(ty0 * TILE);
-    const R edgerx = edgelx + (R)min(TILE, g.nx - tx0 * TILE);
-    const R edgery = edgely + (R)min(TILE, g.ny - ty0 * TILE);
-    for (int i = i0; i < i1; ++i) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = i < np;
+    int k = -1;
+    if (valid) {
         R x = s.x[i], y = s.y[i];
+        const int cx = min(max((int)x, 0), g.nx - 1), cy = min(max((int)y, 0), g.ny - 1);
+        const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
+        const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
+        const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
+        const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
+        const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
+        const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
+        const R idex = R(1) / dex, qtmh = R(0.5) * qbm * dt;
+        const R anx = (R)g.nx, any = (R)g.ny;
+        const int tx0 = cx >> 4, ty0 = cy >> 4;
+        const R edgelx = (R)(tx0 * TILE), edgely = (R)(ty0 * TILE);
+        const R edgerx = edgelx + (R)min(TILE, g.nx - tx0 * TILE);
+        const R edgery = edgely + (R)min(TILE, g.ny - ty0 * TILE);
         R p6 = s.p6[i], p7 = s.p7[i];
         R qtmh1 = qtmh / p7;
         R qtmh2 = qtmh1 * p6;
-        R dxp = x - fx, dyp = y - fy;
+        R dxp = x - (R)cx, dyp = y - (R)cy;
         R amx = R(1) - dxp, amy = R(1) - dyp;
         R dx = gather(ex, amx, amy, dxp, dyp);
         R dy = gather(ey, amx, amy, dxp, dyp);
@@ -396,8 +474,7 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
         s.x[i] = dx;
         s.y[i] = dy;
         int ncx = (int)dx, ncy = (int)dy;
-        // guard against a non-finite or out-of-range position before it is
-        // used as an index
+        // a non-finite or out-of-range position must not become an index
         if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
             atomicMax(err, 1 << 30);
             ncx = min(max(ncx, 0), g.nx - 1);
@@ -405,11 +482,26 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
         }
         int txd = (tx0 + dirx + g.mx1) % g.mx1, tyd = (ty0 + diry + g.my1) % g.my1;
         if ((ncx >> 4) != txd || (ncy >> 4) != tyd) atomicMax(err, 1 + txd + g.mx1 * tyd);
-        int k = cell_key(ncx, ncy, g.mx1);
+        k = cell_key(ncx, ncy, g.mx1);
         newkey[i] = k;
-        slot[i] = atomicAdd(&count[k], 1);
     }
+    // rank in the destination cell: one atomic per distinct key in the wave
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(valid);
+    int myslot = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lk = __shfl(k, leader);
+        const unsigned long long m = __ballot(valid && k == lk);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[lk], __popcll(m));
+        base = __shfl(base, leader);
+        if (valid && k == lk) myslot = base + __popcll(m & ((1ull << lane) - 1ull));
+        todo &= ~m;
+    }
+    if (valid) slot[i] = myslot;
 }
+
 
 // cell keys of unsorted particles (initial load and uploads)
 template <typename R>
@@ -533,13 +625,41 @@ static int deposit_border(qp_ctx *c, const PlaneSet<R> &out, int ncomp, int fold
 }
 
 // species charge into a real-space plane (overwritten), guard-folded
+// deposit tiling per precision: rows per group and staging chunk
+template <typename R> struct DepCfg;
+template <> struct DepCfg<float> {
+    static constexpr int Q_RG = 16, Q_CH = 4096;   // charge: one thread per cell
+    static constexpr int J_RG = 8, J_CH = 2048;    // current: two threads per cell
+};
+template <> struct DepCfg<double> {
+    static constexpr int Q_RG = 8, Q_CH = 2048;
+    static constexpr int J_RG = 4, J_CH = 1024;
+};
+
+// kernels using more than 64 KB of dynamic LDS ask for it once
+template <typename K>
+static int allow_lds(K kernel, size_t bytes, qp_ctx *c)
+{
+    QP_REQUIRE(bytes <= c->max_lds, QP_ERR_UNSUPPORTED, "kernel needs %zu B of LDS (limit %zu)",
+               bytes, c->max_lds);
+    if (bytes > 65536)
+        QP_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)bytes));
+    return QP_OK;
+}
+
 template <typename R>
 static int deposit_q_t(qp_species *sp, R *plane)
 {
     qp_ctx *c = sp->ctx;
+    constexpr int RG = DepCfg<R>::Q_RG, CH = DepCfg<R>::Q_CH;
+    const size_t lds = (3 * (size_t)Stage<CH>::N + 289) * sizeof(R);
+    auto kern = k_deposit_q<R, RG, CH>;
+    int rc = allow_lds(kern, lds, c);
+    if (rc) return rc;
     {
     ProfScope _ps(c, QP_PROF_DEPOSIT_Q, (3.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
-    hipLaunchKernelGGL(k_deposit_q<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
+    hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), plane, c->halo.as<R>());
     }
     QP_LAUNCH_CHECK();
@@ -618,10 +738,11 @@ template <typename R>
 static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
 {
     qp_ctx *c = sp->ctx;
+    if (sp->np == 0) return QP_OK;
     {
     ProfScope _ps(c, QP_PROF_EXTPSI, (6.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
-    hipLaunchKernelGGL(k_extpsi<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
-                       soa<R>(sp->a), sp->off.as<int>(), psi->rs.as<R>(), (R)sp->qbm, (R)dex);
+    hipLaunchKernelGGL(k_extpsi<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np, c->nx, c->ny,
+                       c->ldr, soa<R>(sp->a), psi->rs.as<R>(), (R)sp->qbm, (R)dex);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -639,15 +760,20 @@ static int amjdp_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     out.p[3] = dcu; out.p[4] = dcu + pl;
     out.p[5] = amu; out.p[6] = amu + pl; out.p[7] = amu + 2 * pl;
     const size_t hcs = (size_t)c->ntiles * HALO;
+    constexpr int RG = DepCfg<R>::J_RG, CH = DepCfg<R>::J_CH;
+    const size_t lds = (7 * (size_t)Stage<CH>::N + 8 * 289) * sizeof(R);
+    auto kern = k_amjdp<R, RG, CH>;
+    int rc = allow_lds(kern, lds, c);
+    if (rc) return rc;
     {
     ProfScope _ps(c, QP_PROF_AMJDP, (8.0 * sp->np + 14.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
-    hipLaunchKernelGGL(k_amjdp<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
+    hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
                        psit->rs.as<R>(), pl, out, c->halo.as<R>(), hcs, (R)sp->qbm, (R)sp->dt,
                        (R)dex);
     }
     QP_LAUNCH_CHECK();
-    int rc = deposit_border<R>(c, out, 8, 1);
+    rc = deposit_border<R>(c, out, 8, 1);
     sp->cu->state = sp->dcu->state = sp->amu->state = 0;
     return rc;
 }
@@ -659,13 +785,12 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     qp_ctx *c = sp->ctx;
     int rc = zero_counts(sp);
     if (rc) return rc;
-    {
-    ProfScope _ps(c, QP_PROF_PUSH, (12.0 * sp->np + 6.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
-    hipLaunchKernelGGL(k_push<R>, dim3(c->ntiles), dim3(256), 0, c->stream, geom(c),
-                       soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
-                       psit->rs.as<R>(), c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex,
-                       sp->key.as<int>(), sp->slot.as<int>(), sp->count.as<int>(),
-                       sp->err.as<int>());
+    if (sp->np > 0) {
+        ProfScope _ps(c, QP_PROF_PUSH, (12.0 * sp->np + 6.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
+        hipLaunchKernelGGL(k_push<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np, geom(c),
+                           soa<R>(sp->a), ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(),
+                           c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex, sp->key.as<int>(),
+                           sp->slot.as<int>(), sp->count.as<int>(), sp->err.as<int>());
     }
     QP_LAUNCH_CHECK();
     rc = rebin_tail<R>(sp);
