@@ -28,6 +28,7 @@ constexpr int TCELLS = TILE * TILE;      // cells per tile = threads per particl
 constexpr int HALO = 4 * TILE;           // border nodes of a 17x17 tile block
 constexpr double TWOPI_REF = 6.28318530717959;  // the reference's 2*pi literal
 constexpr double BEAM_FIX = 17592186044416.0;   // 2^44: beam charge fixed point
+constexpr int MAX_BATCH = 8;            // components in one batched transform
 
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
@@ -83,7 +84,7 @@ struct qp_ctx {
     hipStream_t stream;
     qp::DevBuf ffg, ffs;          // Green's function and shape factor [kx][ky], nx*ny
     qp::DevBuf twx, twy;          // twiddles exp(-2 pi i t / 2n), t < 2n
-    qp::DevBuf scr_r, scr_k;      // transform scratch, 3 planes each
+    qp::DevBuf scr_r, scr_k;      // transform scratch, MAX_BATCH planes each
     qp::DevBuf halo;              // deposit tile borders, 8 comps
     qp::DevBuf red;               // reduction partials (double)
     size_t max_lds;               // LDS bytes one workgroup may use
@@ -112,11 +113,42 @@ struct qp_field3d {
 
 namespace qp {
 
+// k-space mode grid: arrays [kx][ky] with pitch ldk, cs elements per
+// component; tables [kx][ky] with pitch ny (kx < nx, ky < ny).
+struct KGrid {
+    int nx, ny, ldk;
+    size_t cs;
+};
+
+// this thread's mode (kx, ky), 0 <= kx <= nx, 0 <= ky <= ny
+__device__ __forceinline__ bool kmode(const KGrid &g, int &kx, int &ky)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int n1 = g.ny + 1;
+    kx = t / n1;
+    ky = t - kx * n1;
+    return kx <= g.nx;
+}
+
+// dnx = 6.28318530717959/real(nx+nx); dkx = dnx*real(j) (fpois2d_lib77.f:1512)
+template <typename R>
+__device__ __forceinline__ R dk(int n, int k)
+{
+    const R d = (R)(TWOPI_REF / (double)(n + n));
+    return d * (R)k;
+}
+
+inline KGrid kgrid(const qp_ctx *c) { return KGrid{c->nx, c->ny, c->ldk, c->kplane}; }
+inline dim3 kgrid_blocks(const qp_ctx *c) { return dim3(((c->nx + 1) * (c->ny + 1) + 255) / 256); }
+
 // ---- transforms (qp_fft.hip) ----
 // component (x, y) transform types of fftrk/fftkr kinds; 0 = sine, 1 = cosine
 void kind_types(int kind, int dim, int *tx, int *ty);
 int fft_r2k(qp_field2d *f, int kind);
 int fft_k2r(qp_field2d *f, int kind);
+// several fields (at most MAX_BATCH components in all) in one set of launches
+int fft_r2k_batch(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf);
+int fft_k2r_batch(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf);
 
 // ---- field kernels (qp_field.hip) ----
 int launch_fill(qp_ctx *c, void *p, size_t n, double v);
@@ -126,6 +158,20 @@ template <typename R> int bfqp_resid(qp_field2d *cu, qp_field2d *dcu, qp_field2d
                                      qp_field2d *bxy, double c, double dex, double *num_den);
 int bfqp_impl(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy,
               double c, double dex, double *resid_out);
+
+constexpr int BFQP_BLOCKS = 1024;   // blocks of the corrector kernels (FP64 partials)
+__global__ void k_sum_partials(const double *partial, int nblk, double *out);
+
+// ---- fused slice-loop kernels (qp_fused.hip) ----
+struct PassFields {
+    qp_field2d *cu, *dcu, *amu, *epw, *epwb, *psit, *bxyz;
+};
+int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
+int fused_pass(const PassFields &f, double dex, bool qp, double *num_den_dev);
+int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
+int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
+int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
+                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex);
 
 // ---- kernel-class timing ----
 int prof_begin(qp_ctx *c, int kind, double bytes);   // returns a slot or -1

@@ -200,8 +200,8 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
     int rc = precision == QP_FP64 ? make_tables<double>(c) : make_tables<float>(c);
     size_t pl = c->rplane > c->kplane ? c->rplane : c->kplane;
-    if (!rc) rc = c->scr_r.alloc(3 * pl * c->esize);
-    if (!rc) rc = c->scr_k.alloc(3 * pl * c->esize);
+    if (!rc) rc = c->scr_r.alloc(MAX_BATCH * pl * c->esize);
+    if (!rc) rc = c->scr_k.alloc(MAX_BATCH * pl * c->esize);
     if (!rc) rc = c->halo.alloc((size_t)8 * c->ntiles * HALO * c->esize);
     if (!rc) rc = c->red.alloc(sizeof(double) * 4096);
     if (rc) { qp_ctx_destroy(c); return rc; }

@@ -9,13 +9,18 @@
 // of fftkr (field2d_class.f03:223): row ny <- row 0.
 //
 // GPU method (not the reference's recursion): two rows of the same type are
-// packed as the real and imaginary part of the 2n-point odd (S) or even (C)
-// extension and transformed by one complex FFT held in LDS (Stockham autosort,
-// radix 4 with one radix-2 pass when log2(2n) is odd).  For the even extension
+// packed as the real and imaginary part of the M = 2n point odd (S) or even
+// (C) extension and transformed by one complex FFT.  For the even extension
 // X(k) = 2 C(k) is real; for the odd one X(k) = -2i S(k); so
 //   C: Re Z = 2 C_a, Im Z = 2 C_b;   S: Re Z = 2 S_b, Im Z = -2 S_a.
-// A pass reads its rows coalesced, transforms them in LDS and writes them
-// back; an LDS-tiled transpose sits between the x and the y pass.
+// The FFT is a Stockham autosort with radix-8 butterflies held in registers
+// (M/8 threads per transform, 8 points each) and one closing radix-2 or
+// radix-4 pass when log2(M) is not a multiple of 3.  The first pass reads the
+// extension straight from memory, the last writes the n+1 outputs straight
+// to memory, and the passes between exchange through LDS; all twiddles are
+// fetched before the first butterfly.  A batch holds up to 8 components of
+// any fields (grid.y); an LDS-tiled transpose sits between the x and the y
+// pass.
 #include "qp_common.h"
 
 namespace qp {
@@ -29,79 +34,129 @@ template <typename R>
 __device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename R>
 __device__ __forceinline__ cplx<R> csub(cplx<R> a, cplx<R> b) { return {a.x - b.x, a.y - b.y}; }
-
-// Transform rows [0, nrows) of every component along their contiguous axis.
-// grid.x = row pairs, grid.y = component.  src may equal dst.  If guard_row
-// >= 0 the result of row 0 is also written to row guard_row.
+// -i z
 template <typename R>
-__global__ void __launch_bounds__(256)
-k_rowfft(const R *__restrict__ src, size_t src_cs, int ld_src, R *dst, size_t dst_cs,
-         int ld_dst, int nrows, int n, int log2m, int types, R scale,
-         const cplx<R> *__restrict__ W, int guard_row)
+__device__ __forceinline__ cplx<R> cmi(cplx<R> a) { return {a.y, -a.x}; }
+
+// forward DFT of 4 points in place (natural order)
+template <typename R>
+__device__ __forceinline__ void dft4(cplx<R> &a0, cplx<R> &a1, cplx<R> &a2, cplx<R> &a3)
 {
+    cplx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmi(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a1 = cadd(t1, t3);
+    a2 = csub(t0, t2);
+    a3 = csub(t1, t3);
+}
+
+// forward DFT of 8 points in place (natural order): decimation in frequency,
+// X[2m] = DFT4(v[n] + v[n+4]), X[2m+1] = DFT4((v[n] - v[n+4]) w^n), w = e^{-i pi/4}
+template <typename R>
+__device__ __forceinline__ void dft8(cplx<R> *v)
+{
+    const R h = (R)0.70710678118654752440;
+    cplx<R> a0 = cadd(v[0], v[4]), a1 = cadd(v[1], v[5]), a2 = cadd(v[2], v[6]),
+            a3 = cadd(v[3], v[7]);
+    cplx<R> c0 = csub(v[0], v[4]), c1 = csub(v[1], v[5]), c2 = csub(v[2], v[6]),
+            c3 = csub(v[3], v[7]);
+    c1 = {(c1.x + c1.y) * h, (c1.y - c1.x) * h};     // * e^{-i pi/4}
+    c2 = cmi(c2);                                      // * e^{-i pi/2}
+    c3 = {(c3.y - c3.x) * h, -(c3.x + c3.y) * h};    // * e^{-3i pi/4}
+    dft4(a0, a1, a2, a3);
+    dft4(c0, c1, c2, c3);
+    v[0] = a0; v[2] = a1; v[4] = a2; v[6] = a3;
+    v[1] = c0; v[3] = c1; v[5] = c2; v[7] = c3;
+}
+
+__device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
+
+template <int L> struct FftShape {
+    static constexpr int M = 1 << L;                // complex points
+    static constexpr int N = M / 2;                 // transform length n
+    static constexpr int TF = M / 8;                // threads per transform
+    static constexpr int BS = TF > 256 ? TF : 256;  // threads per block
+    static constexpr int FB = BS / TF;              // transforms per block
+    static constexpr int NR8 = L / 3;               // radix-8 passes
+    static constexpr int REM = L % 3;               // closing radix-2 (1) / radix-4 (2)
+    static constexpr int LDSM = M + M / 8;          // padded LDS points per transform
+};
+
+template <typename R> struct CompSet {
+    const R *src[8];
+    R *dst[8];
+};
+
+// Transform rows [0, nrows) of every component of the batch along their
+// contiguous axis; row pair p of component blockIdx.y is transform p.  src
+// may equal dst (a block reads its rows completely before writing).  If
+// guard_row >= 0 the result of row 0 is also written to row guard_row.
+template <typename R, int L>
+__global__ void __launch_bounds__(FftShape<L>::BS)
+k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
+           const cplx<R> *__restrict__ W, int guard_row)
+{
+    using S = FftShape<L>;
+    constexpr int M = S::M, n = S::N, TF = S::TF;
     extern __shared__ unsigned char smem_raw[];
-    const int M = 2 * n;
-    cplx<R> *A = reinterpret_cast<cplx<R> *>(smem_raw);
-    cplx<R> *B = A + M;
     const int comp = blockIdx.y;
     const int type = (types >> comp) & 1;   // 0 sine, 1 cosine
-    const int ra = 2 * blockIdx.x;
-    const bool hb = ra + 1 < nrows;
-    const R *sa = src + comp * src_cs + (size_t)ra * ld_src;
-    const R *sb = sa + ld_src;
-    for (int j = threadIdx.x; j < M; j += blockDim.x) {
-        int idx = j <= n ? j : M - j;
-        R va = sa[idx];
-        R vb = hb ? sb[idx] : R(0);
-        if (type == 0) {
-            if (idx == 0 || idx == n) { va = R(0); vb = R(0); }
-            if (j > n) { va = -va; vb = -vb; }
-        }
-        A[j] = {va, vb};
+    const int f = threadIdx.x / TF, t = threadIdx.x % TF;
+    const int ra = 2 * (blockIdx.x * S::FB + f);
+    const bool live = ra < nrows, hb = ra + 1 < nrows;
+    cplx<R> *buf = reinterpret_cast<cplx<R> *>(smem_raw) + f * S::LDSM;
+
+    // twiddles of every pass after the first: pass p (Ns = 8^p) needs
+    // W[r e], e = (t mod Ns) M / (8 Ns); the closing pass W[r j]
+    constexpr int NTW8 = S::NR8 > 1 ? 7 * (S::NR8 - 1) : 1;
+    cplx<R> tw8[NTW8];
+#pragma unroll
+    for (int p = 1; p < S::NR8; ++p) {
+        const int Ns = 1 << (3 * p);
+        const int e = (t & (Ns - 1)) * (M / (8 * Ns));
+#pragma unroll
+        for (int r = 1; r < 8; ++r) tw8[7 * (p - 1) + r - 1] = W[r * e];
     }
-    __syncthreads();
-    cplx<R> *in = A, *out = B;
-    int Ns = 1;
-    if (log2m & 1) {
-        for (int jj = threadIdx.x; jj < M / 2; jj += blockDim.x) {
-            cplx<R> a0 = in[jj], a1 = in[jj + M / 2];
-            out[2 * jj] = cadd(a0, a1);
-            out[2 * jj + 1] = csub(a0, a1);
-        }
-        __syncthreads();
-        cplx<R> *t = in; in = out; out = t;
-        Ns = 2;
+    constexpr int NTWR = S::REM == 2 ? 6 : (S::REM == 1 ? 4 : 1);
+    cplx<R> twr[NTWR];
+    if (S::REM == 2) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 1; r < 4; ++r) twr[3 * s + r - 1] = W[r * (t + s * TF)];
+    } else if (S::REM == 1) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) twr[s] = W[t + s * TF];
     }
-    const int q = M / 4;
-    while (Ns < M) {
-        const int stride = M / (4 * Ns);
-        for (int jj = threadIdx.x; jj < q; jj += blockDim.x) {
-            const int k = jj & (Ns - 1);
-            cplx<R> a0 = in[jj], a1 = in[jj + q], a2 = in[jj + 2 * q], a3 = in[jj + 3 * q];
-            if (k) {
-                a1 = cmul(a1, W[k * stride]);
-                a2 = cmul(a2, W[2 * k * stride]);
-                a3 = cmul(a3, W[3 * k * stride]);
+
+    // first pass: radix 8 on the extension, straight from memory
+    cplx<R> v[8];
+    {
+        const R *sa = cs.src[comp] + (size_t)ra * ld_src;
+        const R *sb = sa + ld_src;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int idx = t + r * TF;
+            const int e = idx <= n ? idx : M - idx;
+            R va = R(0), vb = R(0);
+            if (live) {
+                va = sa[e];
+                if (hb) vb = sb[e];
             }
-            cplx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3);
-            cplx<R> d = csub(a1, a3);
-            cplx<R> t3 = {d.y, -d.x};   // -i * (a1 - a3)
-            const int base = (jj - k) * 4 + k;
-            out[base] = cadd(t0, t2);
-            out[base + Ns] = cadd(t1, t3);
-            out[base + 2 * Ns] = csub(t0, t2);
-            out[base + 3 * Ns] = csub(t1, t3);
+            if (type == 0) {
+                if (e == 0 || e == n) { va = R(0); vb = R(0); }
+                if (idx > n) { va = -va; vb = -vb; }
+            }
+            v[r] = {va, vb};
         }
-        __syncthreads();
-        cplx<R> *t = in; in = out; out = t;
-        Ns *= 4;
+        dft8(v);
     }
+
     const R half = scale * R(0.5);
-    R *da = dst + comp * dst_cs + (size_t)ra * ld_dst;
+    R *da = cs.dst[comp] + (size_t)ra * ld_dst;
     R *db = da + ld_dst;
-    R *dg = guard_row >= 0 && ra == 0 ? dst + comp * dst_cs + (size_t)guard_row * ld_dst : nullptr;
-    for (int k = threadIdx.x; k <= n; k += blockDim.x) {
-        cplx<R> z = in[k];
+    R *dg = (guard_row >= 0 && ra == 0) ? cs.dst[comp] + (size_t)guard_row * ld_dst : nullptr;
+    auto emit = [&](int k, cplx<R> z) {
+        if (!live || k > n) return;
         R oa, ob;
         if (type == 1) {
             oa = z.x * half;
@@ -116,21 +171,78 @@ k_rowfft(const R *__restrict__ src, size_t src_cs, int ld_src, R *dst, size_t ds
         da[k] = oa;
         if (hb) db[k] = ob;
         if (dg) dg[k] = oa;
+    };
+
+    if (S::NR8 == 1 && S::REM == 0) {   // M = 8 is below the supported sizes
+#pragma unroll
+        for (int r = 0; r < 8; ++r) emit(t + r * TF, v[r]);
+        return;
+    }
+    // first pass output (Ns = 1): index 8 t + r
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[pad8(8 * t + r)] = v[r];
+
+    // middle radix-8 passes
+#pragma unroll
+    for (int p = 1; p < S::NR8; ++p) {
+        const int Ns = 1 << (3 * p);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            v[r] = buf[pad8(t + r * TF)];
+            if (r) v[r] = cmul(v[r], tw8[7 * (p - 1) + r - 1]);
+        }
+        dft8(v);
+        if (p == S::NR8 - 1 && S::REM == 0) {
+            // last pass (Ns = M/8): output index t + r M/8 goes to memory
+#pragma unroll
+            for (int r = 0; r < 8; ++r) emit(t + r * TF, v[r]);
+            return;
+        }
+        __syncthreads();
+        const int base = (t / Ns) * Ns * 8 + (t & (Ns - 1));
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[pad8(base + r * Ns)] = v[r];
+    }
+    __syncthreads();
+    if (S::REM == 2) {
+        // closing radix 4 (Ns = M/4): butterflies t and t + TF
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int j = t + s * TF;
+            cplx<R> a0 = buf[pad8(j)];
+            cplx<R> a1 = cmul(buf[pad8(j + M / 4)], twr[3 * s]);
+            cplx<R> a2 = cmul(buf[pad8(j + M / 2)], twr[3 * s + 1]);
+            cplx<R> a3 = cmul(buf[pad8(j + 3 * M / 4)], twr[3 * s + 2]);
+            dft4(a0, a1, a2, a3);
+            emit(j, a0);
+            emit(j + M / 4, a1);
+            emit(j + M / 2, a2);
+        }
+    } else {
+        // closing radix 2 (Ns = M/2): butterflies t + s TF
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int j = t + s * TF;
+            cplx<R> a0 = buf[pad8(j)];
+            cplx<R> a1 = cmul(buf[pad8(j + M / 2)], twr[s]);
+            emit(j, cadd(a0, a1));
+            emit(j + M / 2, csub(a0, a1));
+        }
     }
 }
 
 // dst(c)[col][row] = src(c)[row][col] for row < rows, col < cols
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_transpose(const R *__restrict__ src, size_t src_cs, int ld_src, R *__restrict__ dst,
-            size_t dst_cs, int ld_dst, int rows, int cols)
+k_transpose(CompSet<R> cs, int ld_src, int ld_dst, int rows, int cols)
 {
     __shared__ R tile[32][33];
     const int comp = blockIdx.z;
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const R *s = src + comp * src_cs;
-    R *d = dst + comp * dst_cs;
+    const R *s = cs.src[comp];
+    R *d = cs.dst[comp];
     for (int i = ty; i < 32; i += 8) {
         int r = r0 + i, c = c0 + tx;
         if (r < rows && c < cols) tile[i][tx] = s[(size_t)r * ld_src + c];
@@ -160,83 +272,150 @@ void kind_types(int kind, int dim, int *tx, int *ty)
 
 static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
-template <typename R>
-static int rowfft(qp_ctx *c, const R *src, size_t src_cs, int ld_src, R *dst, size_t dst_cs,
-                  int ld_dst, int dim, int nrows, int n, const int *types, double scale,
-                  const cplx<R> *W, int guard_row)
+template <typename R, int L>
+static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int ld_src,
+                       int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
-    const int M = 2 * n;
-    const size_t lds = 2 * (size_t)M * sizeof(cplx<R>);
+    using S = FftShape<L>;
+    const size_t lds = (size_t)S::FB * S::LDSM * sizeof(cplx<R>);
     QP_REQUIRE(lds <= c->max_lds, QP_ERR_UNSUPPORTED,
-               "transform of length %d needs %zu B of LDS (limit %zu)", n, lds, c->max_lds);
-    int tmask = 0;
-    for (int i = 0; i < dim; ++i) tmask |= (types[i] & 1) << i;
-    dim3 grid((nrows + 1) / 2, dim);
+               "transform of length %d needs %zu B of LDS (limit %zu)", S::N, lds, c->max_lds);
+    auto kern = k_fft_rows<R, L>;
+    if (lds > 65536)
+        QP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    const int pairs = (nrows + 1) / 2;
+    dim3 grid((pairs + S::FB - 1) / S::FB, ncomp);
     {
-    ProfScope _ps(c, QP_PROF_FFT, 2.0 * dim * nrows * (n + 1.0) * sizeof(R));
-    hipLaunchKernelGGL(k_rowfft<R>, grid, dim3(256), lds, c->stream, src, src_cs, ld_src, dst,
-                       dst_cs, ld_dst, nrows, n, ilog2(M), tmask, (R)scale, W, guard_row);
+    ProfScope _ps(c, QP_PROF_FFT, 2.0 * ncomp * nrows * (S::N + 1.0) * sizeof(R));
+    hipLaunchKernelGGL(kern, grid, dim3(S::BS), lds, c->stream, cs, nrows, ld_src, ld_dst, types,
+                       (R)scale, W, guard_row);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
 }
 
 template <typename R>
-static int transpose(qp_ctx *c, const R *src, size_t src_cs, int ld_src, R *dst, size_t dst_cs,
-                     int ld_dst, int dim, int rows, int cols)
+static int rowfft(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int n, int ld_src,
+                  int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
-    dim3 grid((cols + 31) / 32, (rows + 31) / 32, dim);
+    switch (ilog2(2 * n)) {
+    case 5: return rows_launch<R, 5>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 6: return rows_launch<R, 6>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 7: return rows_launch<R, 7>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 8: return rows_launch<R, 8>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 9: return rows_launch<R, 9>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 10: return rows_launch<R, 10>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 11: return rows_launch<R, 11>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 12: return rows_launch<R, 12>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 13: return rows_launch<R, 13>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    }
+    set_error("transform length %d unsupported", n);
+    return QP_ERR_UNSUPPORTED;
+}
+
+template <typename R>
+static int transpose(qp_ctx *c, const CompSet<R> &cs, int ncomp, int ld_src, int ld_dst,
+                     int rows, int cols)
+{
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32, ncomp);
     {
-    ProfScope _ps(c, QP_PROF_TRANSPOSE, 2.0 * dim * (double)rows * cols * sizeof(R));
-    hipLaunchKernelGGL(k_transpose<R>, grid, dim3(256), 0, c->stream, src, src_cs, ld_src, dst,
-                       dst_cs, ld_dst, rows, cols);
+    ProfScope _ps(c, QP_PROF_TRANSPOSE, 2.0 * ncomp * (double)rows * cols * sizeof(R));
+    hipLaunchKernelGGL(k_transpose<R>, grid, dim3(256), 0, c->stream, cs, ld_src, ld_dst, rows,
+                       cols);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
 }
 
+// components of a batch of fields: plane pointers and type masks
 template <typename R>
-static int r2k_t(qp_field2d *f, int kind)
+static int gather_comps(qp_field2d *const *fs, const int *kinds, int nf, R **rs, R **ks,
+                        int *mx, int *my)
 {
-    qp_ctx *c = f->ctx;
-    int tx[3], ty[3];
-    kind_types(kind, f->dim, tx, ty);
-    R *rs = f->rs.as<R>(), *ks = f->ks.as<R>(), *sr = c->scr_r.as<R>();
+    int nc = 0;
+    *mx = *my = 0;
+    for (int i = 0; i < nf; ++i) {
+        qp_field2d *f = fs[i];
+        int tx[3], ty[3];
+        kind_types(kinds[i], f->dim, tx, ty);
+        for (int d = 0; d < f->dim; ++d) {
+            QP_REQUIRE(nc < MAX_BATCH, QP_ERR_ARG, "transform batch exceeds %d components",
+                       MAX_BATCH);
+            rs[nc] = f->rs.as<R>() + d * f->ctx->rplane;
+            ks[nc] = f->ks.as<R>() + d * f->ctx->kplane;
+            *mx |= tx[d] << nc;
+            *my |= ty[d] << nc;
+            ++nc;
+        }
+    }
+    return nc;
+}
+
+template <typename R>
+static int r2k_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
+{
+    R *rs[MAX_BATCH], *ks[MAX_BATCH];
+    int mx, my;
+    const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
+    if (nc <= 0) return nc;
+    R *sr = c->scr_r.as<R>();
+    CompSet<R> a{}, b{}, y{};
+    for (int i = 0; i < nc; ++i) {
+        a.src[i] = rs[i];
+        a.dst[i] = sr + i * c->rplane;
+        b.src[i] = a.dst[i];
+        b.dst[i] = ks[i];
+        y.src[i] = ks[i];
+        y.dst[i] = ks[i];
+    }
     const double ani = 1.0 / ((double)c->nx * (double)c->ny);
-    int rc = rowfft<R>(c, rs, c->rplane, c->ldr, sr, c->rplane, c->ldr, f->dim, c->ny + 1,
-                       c->nx, tx, ani, c->twx.as<cplx<R>>(), -1);
-    if (!rc) rc = transpose<R>(c, sr, c->rplane, c->ldr, ks, c->kplane, c->ldk, f->dim,
-                               c->ny + 1, c->nx + 1);
-    if (!rc) rc = rowfft<R>(c, ks, c->kplane, c->ldk, ks, c->kplane, c->ldk, f->dim, c->nx + 1,
-                            c->ny, ty, 1.0, c->twy.as<cplx<R>>(), -1);
+    int rc = rowfft<R>(c, a, nc, c->ny + 1, c->nx, c->ldr, c->ldr, mx, ani,
+                       c->twx.as<cplx<R>>(), -1);
+    if (!rc) rc = transpose<R>(c, b, nc, c->ldr, c->ldk, c->ny + 1, c->nx + 1);
+    if (!rc) rc = rowfft<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
+                            c->twy.as<cplx<R>>(), -1);
     return rc;
 }
 
 template <typename R>
-static int k2r_t(qp_field2d *f, int kind)
+static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
 {
-    qp_ctx *c = f->ctx;
-    int tx[3], ty[3];
-    kind_types(kind, f->dim, tx, ty);
-    R *rs = f->rs.as<R>(), *ks = f->ks.as<R>(), *sk = c->scr_k.as<R>();
-    int rc = rowfft<R>(c, ks, c->kplane, c->ldk, sk, c->kplane, c->ldk, f->dim, c->nx + 1,
-                       c->ny, ty, 2.0, c->twy.as<cplx<R>>(), -1);
-    if (!rc) rc = transpose<R>(c, sk, c->kplane, c->ldk, rs, c->rplane, c->ldr, f->dim,
-                               c->nx + 1, c->ny + 1);
+    R *rs[MAX_BATCH], *ks[MAX_BATCH];
+    int mx, my;
+    const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
+    if (nc <= 0) return nc;
+    R *sk = c->scr_k.as<R>();
+    CompSet<R> y{}, b{}, a{};
+    for (int i = 0; i < nc; ++i) {
+        y.src[i] = ks[i];
+        y.dst[i] = sk + i * c->kplane;
+        b.src[i] = y.dst[i];
+        b.dst[i] = rs[i];
+        a.src[i] = rs[i];
+        a.dst[i] = rs[i];
+    }
+    int rc = rowfft<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,
+                       c->twy.as<cplx<R>>(), -1);
+    if (!rc) rc = transpose<R>(c, b, nc, c->ldk, c->ldr, c->nx + 1, c->ny + 1);
     // rows 0..ny-1; row ny receives row 0 (guard copy, PPNCGUARD2L nvp = 1)
-    if (!rc) rc = rowfft<R>(c, rs, c->rplane, c->ldr, rs, c->rplane, c->ldr, f->dim, c->ny,
-                            c->nx, tx, 2.0, c->twx.as<cplx<R>>(), c->ny);
+    if (!rc) rc = rowfft<R>(c, a, nc, c->ny, c->nx, c->ldr, c->ldr, mx, 2.0,
+                            c->twx.as<cplx<R>>(), c->ny);
     return rc;
 }
 
-int fft_r2k(qp_field2d *f, int kind)
+int fft_r2k_batch(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
 {
-    return f->ctx->prec == QP_FP64 ? r2k_t<double>(f, kind) : r2k_t<float>(f, kind);
+    return c->prec == QP_FP64 ? r2k_t<double>(c, fs, kinds, nf) : r2k_t<float>(c, fs, kinds, nf);
 }
 
-int fft_k2r(qp_field2d *f, int kind)
+int fft_k2r_batch(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
 {
-    return f->ctx->prec == QP_FP64 ? k2r_t<double>(f, kind) : k2r_t<float>(f, kind);
+    return c->prec == QP_FP64 ? k2r_t<double>(c, fs, kinds, nf) : k2r_t<float>(c, fs, kinds, nf);
 }
+
+int fft_r2k(qp_field2d *f, int kind) { return fft_r2k_batch(f->ctx, &f, &kind, 1); }
+
+int fft_k2r(qp_field2d *f, int kind) { return fft_k2r_batch(f->ctx, &f, &kind, 1); }
 
 }  // namespace qp

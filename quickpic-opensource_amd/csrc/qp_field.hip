@@ -137,35 +137,12 @@ int copyguard(qp_field2d *f)
 // of its reference routine; one thread per mode (kx, ky), 0 <= kx <= nx,
 // 0 <= ky <= ny.
 // ------------------------------------------------------------------------
-struct KGrid {
-    int nx, ny, ldk;
-    size_t cs;
-};
-
-template <typename R>
-__device__ __forceinline__ bool kmode(const KGrid &g, int &kx, int &ky)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int n1 = g.ny + 1;
-    kx = t / n1;
-    ky = t - kx * n1;
-    return kx <= g.nx;
-}
-
-template <typename R>
-__device__ __forceinline__ R dk(int n, int k)
-{
-    // dnx = 6.28318530717959/real(nx+nx); dkx = dnx*real(j)
-    const R d = (R)(TWOPI_REF / (double)(n + n));
-    return d * (R)k;
-}
-
 // MPPDIVFD2 fft2d_lib77.f:9170-9229
 template <typename R>
 __global__ void k_div(KGrid g, const R *f, R *df)
 {
     int kx, ky;
-    if (!kmode<R>(g, kx, ky)) return;
+    if (!kmode(g, kx, ky)) return;
     size_t o = (size_t)kx * g.ldk + ky;
     R v = R(0);
     if (kx > 0 && kx < g.nx && ky > 0 && ky < g.ny)
@@ -178,7 +155,7 @@ template <typename R>
 __global__ void k_grad(KGrid g, const R *df, R *f)
 {
     int kx, ky;
-    if (!kmode<R>(g, kx, ky)) return;
+    if (!kmode(g, kx, ky)) return;
     size_t o = (size_t)kx * g.ldk + ky;
     R a = R(0), b = R(0);
     if (kx > 0 && kx < g.nx && ky > 0 && ky < g.ny) {
@@ -195,7 +172,7 @@ template <typename R>
 __global__ void k_scale_tab(KGrid g, const R *tab, const R *q, R *out)
 {
     int kx, ky;
-    if (!kmode<R>(g, kx, ky)) return;
+    if (!kmode(g, kx, ky)) return;
     size_t o = (size_t)kx * g.ldk + ky;
     R v = R(0);
     if (kx > 0 && kx < g.nx && ky > 0 && ky < g.ny) v = tab[(size_t)kx * g.ny + ky] * q[o];
@@ -207,7 +184,7 @@ template <typename R>
 __global__ void k_elf(KGrid g, const R *ffg, const R *ffs, const R *q, R *fxy)
 {
     int kx, ky;
-    if (!kmode<R>(g, kx, ky)) return;
+    if (!kmode(g, kx, ky)) return;
     size_t o = (size_t)kx * g.ldk + ky;
     R a = R(0), b = R(0);
     if (kx > 0 && kx < g.nx && ky > 0 && ky < g.ny) {
@@ -226,7 +203,7 @@ template <typename R>
 __global__ void k_bf(KGrid g, const R *ffg, const R *ffs, const R *cu, R *bxy)
 {
     int kx, ky;
-    if (!kmode<R>(g, kx, ky)) return;
+    if (!kmode(g, kx, ky)) return;
     size_t o = (size_t)kx * g.ldk + ky;
     R b1 = R(0), b2 = R(0), b3 = R(0);
     if (kx < g.nx && ky < g.ny) {
@@ -338,10 +315,6 @@ __global__ void k_sum_partials(const double *partial, int nblk, double *out)
     if (threadIdx.x == 0) { out[0] = s[0][0]; out[1] = s[1][0]; }
 }
 
-static KGrid kgrid(qp_ctx *c) { return KGrid{c->nx, c->ny, c->ldk, c->kplane}; }
-static dim3 kgrid_blocks(qp_ctx *c) { return dim3(((c->nx + 1) * (c->ny + 1) + 255) / 256); }
-
-constexpr int BFQP_BLOCKS = 1024;
 
 template <typename R>
 int bfqp_resid(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy, double cc,

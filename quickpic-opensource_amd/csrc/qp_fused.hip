@@ -1,0 +1,303 @@
+// Fused field kernels of the xi-slice loop.  Each one restates a fixed
+// sequence of field2d calls of go_simulation (simulation_class.f03:643-736)
+// mode by mode or node by node, with the same operations in the same order,
+// so its results are those of the separate calls; it only saves the launches
+// and the intermediate planes.
+#include "qp_common.h"
+
+namespace qp {
+
+// pot then grad (simulation_class.f03:659-660): psi = g q (MPPOTPD2,
+// fpois2d_lib77.f:1701-1779), fxy = (dkx psi, dky psi) (MPPGRADFD2,
+// fft2d_lib77.f:9231-9295); both zero off 0 < kx < nx, 0 < ky < ny.
+template <typename R>
+__global__ void k_psi_fxy(KGrid g, const R *__restrict__ ffg, const R *__restrict__ q, R *psi,
+                          R *fxy)
+{
+    int kx, ky;
+    if (!kmode(g, kx, ky)) return;
+    const size_t o = (size_t)kx * g.ldk + ky;
+    R p = R(0), a = R(0), b = R(0);
+    if (kx > 0 && kx < g.nx && ky > 0 && ky < g.ny) {
+        p = ffg[(size_t)kx * g.ny + ky] * q[o];
+        a = dk<R>(g.nx, kx) * p;
+        b = dk<R>(g.ny, ky) * p;
+    }
+    psi[o] = p;
+    fxy[o] = a;
+    fxy[g.cs + o] = b;
+}
+
+// One corrector pass in k space (simulation_class.f03:697-709 then the
+// B assembly of :677-680 that opens the next pass or follows the loop):
+//   QP:  bfqp(cu, dcu, amu, epw)   PBPOISD22N_QP isign = 1 (epw, dcu updated)
+//   psit = pot(div(cu)) * (-dex)   MPPDIVFD2, MPPOTPD2
+//   b    = bf(cu)                  MPPBBPOISD23
+//   bxyz = (-dex epw_y - epwb_y, dex epw_x + epwb_x, dex b_z)
+// Without QP (the predictor) epw is read as it stands.  With QP the FP64
+// squared change and squared new value of epw are summed per block for the
+// corrector's stopping test.
+template <typename R, bool QP>
+__global__ void __launch_bounds__(256)
+k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *__restrict__ cu,
+       R *dcu, const R *__restrict__ amu, R *epw, const R *__restrict__ epwb, R *psit, R *bxyz,
+       R aa, R idex, R dex, double *partial)
+{
+    __shared__ double s_num[256], s_den[256];
+    double num = 0.0, den = 0.0;
+    const size_t cs = g.cs;
+    const int total = (g.nx + 1) * (g.ny + 1);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int kx = t / (g.ny + 1), ky = t - kx * (g.ny + 1);
+        const size_t o = (size_t)kx * g.ldk + ky;
+        const bool tab = kx < g.nx && ky < g.ny;
+        const size_t tt = (size_t)kx * g.ny + ky;
+        const size_t ti = tab ? tt : 0;
+        const R fg = tab ? ffg[ti] : R(0), fs = tab ? ffs[ti] : R(0);
+        const R dkx = dk<R>(g.nx, kx), dky = dk<R>(g.ny, ky);
+        const R c1 = cu[o], c2 = cu[cs + o], c3 = cu[2 * cs + o];
+        R e1 = epw[o], e2 = epw[cs + o];
+        if (QP) {
+            const R old1 = e1, old2 = e2;
+            R n1 = R(0), n2 = R(0);
+            if (tab) {
+                R at2 = R(1) * fg / (R(1) + aa * fg);
+                R at1 = at2 * fs;
+                if (kx > 0 && ky > 0) {
+                    R d1 = -dcu[o] + dkx * amu[o] - dky * amu[2 * cs + o];
+                    R aat1 = d1 - dkx * c3 * idex;
+                    R d2 = -dcu[cs + o] - dkx * amu[2 * cs + o] + dky * amu[cs + o];
+                    R aat2 = d2 - dky * c3 * idex;
+                    dcu[o] = d1;
+                    dcu[cs + o] = d2;
+                    n1 = at1 * aat1 + aa * at2 * old1;
+                    n2 = at1 * aat2 + aa * at2 * old2;
+                } else if (kx > 0) {
+                    R d1 = -dcu[o] + dkx * amu[o];
+                    R d2 = -dcu[cs + o] - dkx * amu[2 * cs + o];
+                    dcu[o] = d1;
+                    dcu[cs + o] = d2;
+                    n2 = at1 * d2 + aa * at2 * old2;
+                } else if (ky > 0) {
+                    R d1 = -dcu[o] - dky * amu[2 * cs + o];
+                    R d2 = -dcu[cs + o] + dky * amu[cs + o];
+                    dcu[o] = d1;
+                    dcu[cs + o] = d2;
+                    n1 = at1 * d1 + aa * at2 * old1;
+                }
+            }
+            epw[o] = n1;
+            epw[cs + o] = n2;
+            const double q1 = (double)n1 - (double)old1, q2 = (double)n2 - (double)old2;
+            num += q1 * q1 + q2 * q2;
+            den += (double)n1 * n1 + (double)n2 * n2;
+            e1 = n1;
+            e2 = n2;
+        }
+        // psit: div (MPPDIVFD2) -> pot (MPPOTPD2) -> * (-dex)
+        R ps = R(0);
+        if (kx > 0 && kx < g.nx && ky > 0 && ky < g.ny) ps = fg * (-(dkx * c1 + dky * c2));
+        psit[o] = ps * (-dex);
+        // bf (MPPBBPOISD23, ci = 1): only its z component survives the assembly
+        R b3 = R(0);
+        if (tab) {
+            const R at1 = R(1) * fg * fs;
+            if (kx > 0 && ky > 0) b3 = (dkx * at1) * c2 - (dky * at1) * c1;
+            else if (kx > 0) b3 = dkx * at1 * c2;
+            else if (ky > 0) b3 = -(dky * at1) * c1;
+        }
+        bxyz[o] = e2 * (-dex) - epwb[cs + o];
+        bxyz[cs + o] = e1 * dex + epwb[o];
+        bxyz[2 * cs + o] = b3 * dex;
+    }
+    if (!QP) return;
+    s_num[threadIdx.x] = num;
+    s_den[threadIdx.x] = den;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_num[threadIdx.x] += s_num[threadIdx.x + s];
+            s_den[threadIdx.x] += s_den[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s_num[0];
+        partial[2 * blockIdx.x + 1] = s_den[0];
+    }
+}
+
+// after the loop (simulation_class.f03:725-726): dcu *= dxi; cu(1:2) -= dcu(1:2)
+template <typename R>
+__global__ void k_predict_cu(size_t n, size_t cs, R *cu, R *dcu, R dxi)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const R d1 = dcu[e] * dxi, d2 = dcu[cs + e] * dxi;
+        dcu[e] = d1;
+        dcu[cs + e] = d2;
+        cu[e] = cu[e] - d1;
+        cu[cs + e] = cu[cs + e] - d2;
+    }
+}
+
+// (simulation_class.f03:721-723) fxy = -fxy; fxy(1) += bxyz(2); fxy(2) -= bxyz(1)
+template <typename R>
+__global__ void k_efield(size_t n, size_t cs, R *fxy, const R *__restrict__ bxyz)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const R f1 = fxy[e] * R(-1), f2 = fxy[cs + e] * R(-1);
+        fxy[e] = f1 + bxyz[cs + e];
+        fxy[cs + e] = f2 - bxyz[e];
+    }
+}
+
+// slice writes (simulation_class.f03:731-735): fxy *= dex, copied to
+// bexyz(1:2); psit to bexyz(3); bxyz(1:2) *= dex; bxyz to bbxyz(1:3)
+template <typename R>
+__global__ void k_slice_write(size_t n, size_t cs, R *fxy, const R *__restrict__ psit, R *bxyz,
+                              R *be1, R *be2, R *be3, R *bb1, R *bb2, R *bb3, R dex)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const R f1 = fxy[e] * dex, f2 = fxy[cs + e] * dex;
+        const R b1 = bxyz[e] * dex, b2 = bxyz[cs + e] * dex;
+        fxy[e] = f1;
+        fxy[cs + e] = f2;
+        bxyz[e] = b1;
+        bxyz[cs + e] = b2;
+        be1[e] = f1;
+        be2[e] = f2;
+        be3[e] = psit[e];
+        bb1[e] = b1;
+        bb2[e] = b2;
+        bb3[e] = bxyz[2 * cs + e];
+    }
+}
+
+static dim3 flat_grid(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return dim3((unsigned)(b ? b : 1));
+}
+
+template <typename R>
+static int psi_fxy_t(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy)
+{
+    qp_ctx *c = q->ctx;
+    {
+    ProfScope _ps(c, QP_PROF_KSPACE, 5.0 * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R));
+    hipLaunchKernelGGL(k_psi_fxy<R>, kgrid_blocks(c), dim3(256), 0, c->stream, kgrid(c),
+                       c->ffg.as<R>(), q->ks.as<R>(), psi->ks.as<R>(), fxy->ks.as<R>());
+    }
+    QP_LAUNCH_CHECK();
+    psi->state = fxy->state = 1;
+    return QP_OK;
+}
+
+int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy)
+{
+    return q->ctx->prec == QP_FP64 ? psi_fxy_t<double>(q, psi, fxy)
+                                   : psi_fxy_t<float>(q, psi, fxy);
+}
+
+template <typename R>
+static int pass_t(const PassFields &f, double dex, bool qp, double *num_den_dev)
+{
+    qp_ctx *c = f.cu->ctx;
+    double *part = c->red.as<double>();
+    // cu 3, dcu 2+2, amu 3, epw 2+2, epwb 2, tables 2, psit 1, bxyz 3
+    const double planes = qp ? 22.0 : 12.0;
+    {
+    ProfScope _ps(c, QP_PROF_KSPACE, planes * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R));
+    auto kern = qp ? k_pass<R, true> : k_pass<R, false>;
+    hipLaunchKernelGGL(kern, dim3(BFQP_BLOCKS), dim3(256), 0, c->stream, kgrid(c),
+                       c->ffg.as<R>(), c->ffs.as<R>(), f.cu->ks.as<R>(), f.dcu->ks.as<R>(),
+                       f.amu->ks.as<R>(), f.epw->ks.as<R>(), f.epwb->ks.as<R>(),
+                       f.psit->ks.as<R>(), f.bxyz->ks.as<R>(), (R)(dex * dex), (R)(1.0 / dex),
+                       (R)dex, part);
+    }
+    QP_LAUNCH_CHECK();
+    if (qp) {
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, part, BFQP_BLOCKS,
+                           num_den_dev);
+        QP_LAUNCH_CHECK();
+        f.epw->state = f.dcu->state = 1;
+    }
+    f.psit->state = f.bxyz->state = 1;
+    return QP_OK;
+}
+
+int fused_pass(const PassFields &f, double dex, bool qp, double *num_den_dev)
+{
+    return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, num_den_dev)
+                                      : pass_t<float>(f, dex, qp, num_den_dev);
+}
+
+template <typename R>
+static int predict_cu_t(qp_field2d *cu, qp_field2d *dcu, double dxi)
+{
+    qp_ctx *c = cu->ctx;
+    const size_t n = c->kplane;
+    {
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 8.0 * n * sizeof(R));
+    hipLaunchKernelGGL(k_predict_cu<R>, flat_grid(n), dim3(256), 0, c->stream, n, n,
+                       cu->ks.as<R>(), dcu->ks.as<R>(), (R)dxi);
+    }
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi)
+{
+    return cu->ctx->prec == QP_FP64 ? predict_cu_t<double>(cu, dcu, dxi)
+                                    : predict_cu_t<float>(cu, dcu, dxi);
+}
+
+template <typename R>
+static int efield_t(qp_field2d *fxy, const qp_field2d *bxyz)
+{
+    qp_ctx *c = fxy->ctx;
+    const size_t n = c->rplane;
+    {
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 6.0 * n * sizeof(R));
+    hipLaunchKernelGGL(k_efield<R>, flat_grid(n), dim3(256), 0, c->stream, n, n,
+                       fxy->rs.as<R>(), bxyz->rs.as<R>());
+    }
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz)
+{
+    return fxy->ctx->prec == QP_FP64 ? efield_t<double>(fxy, bxyz) : efield_t<float>(fxy, bxyz);
+}
+
+template <typename R>
+static int slice_write_t(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
+                         qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex)
+{
+    qp_ctx *c = fxy->ctx;
+    const size_t n = c->rplane, st = (size_t)(c->nz + 1) * n, z = (size_t)(lpos - 1) * n;
+    R *be = bexyz->buf.as<R>() + z, *bb = bbxyz->buf.as<R>() + z;
+    {
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 16.0 * n * sizeof(R));
+    hipLaunchKernelGGL(k_slice_write<R>, flat_grid(n), dim3(256), 0, c->stream, n, n,
+                       fxy->rs.as<R>(), psit->rs.as<R>(), bxyz->rs.as<R>(), be, be + st,
+                       be + 2 * st, bb, bb + st, bb + 2 * st, (R)dex);
+    }
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
+                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex)
+{
+    return fxy->ctx->prec == QP_FP64
+               ? slice_write_t<double>(fxy, psit, bxyz, bexyz, bbxyz, lpos, dex)
+               : slice_write_t<float>(fxy, psit, bxyz, bexyz, bbxyz, lpos, dex);
+}
+
+}  // namespace qp

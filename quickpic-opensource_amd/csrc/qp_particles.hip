@@ -90,15 +90,44 @@ __device__ __forceinline__ int halo_index(int i, int j)
     return 49 + (j - 1);
 }
 
-template <typename R> struct PlaneSet {
+// Where a deposit's node sums go.  For component c and node value v:
+//   t = v (+ inner[c][node]);  t *= scale;  p[c][node] = (outer[c][node] +) t
+//   p2[c][node] = v * scale2                                (when p2[c] is set)
+// With null inner/outer and unit scales this is a plain overwrite; the other
+// settings restate the field additions and scalings that species2d and the
+// slice loop apply right after a deposit (species2d_class.f03:166-218,
+// simulation_class.f03:682-685), in the same order.
+template <typename R> struct DepOut {
     R *p[8];
+    const R *inner[8];
+    const R *outer[8];
+    R *p2[8];
+    R scale, scale2;
 };
+
+template <typename R>
+__device__ __forceinline__ void dep_store(const DepOut<R> &o, int c, size_t idx, R v)
+{
+    R t = v;
+    if (o.inner[c]) t = t + o.inner[c][idx];
+    t = t * o.scale;
+    if (o.p2[c]) o.p2[c][idx] = v * o.scale2;
+    o.p[c][idx] = o.outer[c] ? o.outer[c][idx] + t : t;
+}
+
+template <typename R> static DepOut<R> dep_plain(R *const *planes, int n)
+{
+    DepOut<R> o{};
+    for (int c = 0; c < n; ++c) o.p[c] = planes[c];
+    o.scale = o.scale2 = R(1);
+    return o;
+}
 
 // Sum the tile-border contributions of every border node in a fixed order;
 // with fold, row ny is added into row 0 and cleared (PPNACGUARD2L nvp = 1).
 template <typename R>
 __global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t halo_cs,
-                                 PlaneSet<R> out, int fold)
+                                 DepOut<R> out, int fold)
 {
     const int comp = blockIdx.y;
     const int rows_n = (g.my1 + 1) * (g.nx + 1);
@@ -137,7 +166,7 @@ __global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t 
     if (fold && Y == g.ny) v = R(0);
     else if (fold && Y == 0) { v = node_sum(X, 0); v = v + node_sum(X, g.ny); }
     else v = node_sum(X, Y);
-    out.p[comp][(size_t)Y * g.ldr + X] = v;
+    dep_store(out, comp, (size_t)Y * g.ldr + X, v);
 }
 
 // LDS slot of staged particle p: one pad word every 32 entries spreads the
@@ -151,14 +180,14 @@ template <int CH> struct Stage {
 // Interior nodes of the tile's 17x17 block to the output plane, border nodes
 // to the halo buffer (summed over tiles by k_deposit_border).
 template <typename R>
-__device__ __forceinline__ void emit_nodes(const R *nacc, R *plane, int ldr, R *halo,
-                                           int tile, int tx, int ty)
+__device__ __forceinline__ void emit_nodes(const R *nacc, const DepOut<R> &out, int c, int ldr,
+                                           R *halo, int tile, int tx, int ty)
 {
     for (int n = threadIdx.x; n < 17 * 17; n += TCELLS) {
         const int i = n % 17, j = n / 17;
         const R v = nacc[n];
         if (i >= 1 && i < TILE && j >= 1 && j < TILE)
-            plane[(size_t)(ty * TILE + j) * ldr + tx * TILE + i] = v;
+            dep_store(out, c, (size_t)(ty * TILE + j) * ldr + tx * TILE + i, v);
         else
             halo[(size_t)tile * HALO + halo_index(i, j)] = v;
     }
@@ -176,7 +205,7 @@ __device__ __forceinline__ void emit_nodes(const R *nacc, R *plane, int ldr, R *
 // PPGPPOST2L (part2d_lib77.f:194-309): q(n,m) += qm (1-dx)(1-dy) etc.
 template <typename R, int RG, int CH>
 __global__ void __launch_bounds__(256)
-k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, R *plane, R *halo)
+k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R *halo)
 {
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
     extern __shared__ unsigned char smem_raw[];
@@ -233,7 +262,7 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, R *plane, R *halo
         }
     }
     __syncthreads();
-    emit_nodes(nacc, plane, g.ldr, halo, tile, tx, ty);
+    emit_nodes(nacc, out, 0, g.ldr, halo, tile, tx, ty);
 }
 
 // WPGPSIPOST2L_QP (part2d_lib77.f:2207-2284), one thread per particle
@@ -261,7 +290,7 @@ k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R d
 template <typename R, int RG, int CH>
 __global__ void __launch_bounds__(256)
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
-        const R *psit, size_t cs, PlaneSet<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex)
+        const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex)
 {
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
     extern __shared__ unsigned char smem_raw[];
@@ -383,7 +412,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-        emit_nodes(nacc + c * 289, out.p[c], g.ldr, halo + c * halo_cs, tile, tx, ty);
+        emit_nodes(nacc + c * 289, out, c, g.ldr, halo + c * halo_cs, tile, tx, ty);
 }
 
 // PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
@@ -611,7 +640,7 @@ static int rebin_unsorted(qp_species *sp)
 }
 
 template <typename R>
-static int deposit_border(qp_ctx *c, const PlaneSet<R> &out, int ncomp, int fold)
+static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
 {
     TileGeom g = geom(c);
     int n = (c->my1 + 1) * (c->nx + 1) + (c->mx1 + 1) * c->my1 * (TILE - 1);
@@ -649,7 +678,7 @@ static int allow_lds(K kernel, size_t bytes, qp_ctx *c)
 }
 
 template <typename R>
-static int deposit_q_t(qp_species *sp, R *plane)
+static int deposit_q_out(qp_species *sp, const DepOut<R> &out)
 {
     qp_ctx *c = sp->ctx;
     constexpr int RG = DepCfg<R>::Q_RG, CH = DepCfg<R>::Q_CH;
@@ -660,12 +689,16 @@ static int deposit_q_t(qp_species *sp, R *plane)
     {
     ProfScope _ps(c, QP_PROF_DEPOSIT_Q, (3.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
-                       soa<R>(sp->a), sp->off.as<int>(), plane, c->halo.as<R>());
+                       soa<R>(sp->a), sp->off.as<int>(), out, c->halo.as<R>());
     }
     QP_LAUNCH_CHECK();
-    PlaneSet<R> ps{};
-    ps.p[0] = plane;
-    return deposit_border<R>(c, ps, 1, 1);
+    return deposit_border<R>(c, out, 1, 1);
+}
+
+template <typename R>
+static int deposit_q_t(qp_species *sp, R *plane)
+{
+    return deposit_q_out<R>(sp, dep_plain<R>(&plane, 1));
 }
 
 // dist2d_000 (fdist2d_class.f03:212-316), one rank: x in [1,nx-1),
@@ -749,16 +782,11 @@ static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
 }
 
 template <typename R>
-static int amjdp_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
-                   const qp_field2d *psit, double dex)
+static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                     const qp_field2d *psit, double dex, const DepOut<R> &out)
 {
     qp_ctx *c = sp->ctx;
-    PlaneSet<R> out{};
-    R *cu = sp->cu->rs.as<R>(), *dcu = sp->dcu->rs.as<R>(), *amu = sp->amu->rs.as<R>();
     const size_t pl = c->rplane;
-    out.p[0] = cu; out.p[1] = cu + pl; out.p[2] = cu + 2 * pl;
-    out.p[3] = dcu; out.p[4] = dcu + pl;
-    out.p[5] = amu; out.p[6] = amu + pl; out.p[7] = amu + 2 * pl;
     const size_t hcs = (size_t)c->ntiles * HALO;
     constexpr int RG = DepCfg<R>::J_RG, CH = DepCfg<R>::J_CH;
     const size_t lds = (7 * (size_t)Stage<CH>::N + 8 * 289) * sizeof(R);
@@ -773,7 +801,18 @@ static int amjdp_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                        (R)dex);
     }
     QP_LAUNCH_CHECK();
-    rc = deposit_border<R>(c, out, 8, 1);
+    return deposit_border<R>(c, out, 8, 1);
+}
+
+// the species' own cu, dcu, amu (overwritten)
+template <typename R>
+static int amjdp_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                   const qp_field2d *psit, double dex)
+{
+    const size_t pl = sp->ctx->rplane;
+    R *cu = sp->cu->rs.as<R>(), *dcu = sp->dcu->rs.as<R>(), *amu = sp->amu->rs.as<R>();
+    R *planes[8] = {cu, cu + pl, cu + 2 * pl, dcu, dcu + pl, amu, amu + pl, amu + 2 * pl};
+    int rc = amjdp_out<R>(sp, ef, bf, psit, dex, dep_plain<R>(planes, 8));
     sp->cu->state = sp->dcu->state = sp->amu->state = 0;
     return rc;
 }
@@ -1030,4 +1069,62 @@ int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d 
     return DISPATCH(sp->ctx, push_t, sp, ef, bf, psit, dex, false);
 }
 int species_check(qp_species *sp) { return check_err(sp, "pmove"); }
+
+// qdp_species2d with the slice's additions fused into the deposit
+// (species2d_class.f03:166-185): this%q = deposit; q = this%q + q;
+// q = qn + q.  first: q holds nothing yet (the slice loop zeroed it).
+template <typename R>
+static int qdp_fused_t(qp_species *sp, qp_field2d *q, bool first)
+{
+    R *qp = q->rs.as<R>();
+    DepOut<R> o{};
+    o.p[0] = qp;
+    o.inner[0] = first ? nullptr : qp;
+    o.outer[0] = sp->qn->rs.as<R>();
+    o.p2[0] = sp->q->rs.as<R>();
+    o.scale = o.scale2 = R(1);
+    int rc = deposit_q_out<R>(sp, o);
+    sp->q->state = 0;
+    sp->q->gcells = 1;
+    q->state = 0;
+    return rc;
+}
+
+int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first)
+{
+    return DISPATCH(sp->ctx, qdp_fused_t, sp, q, first);
+}
+
+// amjdp_species2d plus the slice loop's scaling (species2d_class.f03:187-218,
+// simulation_class.f03:682-685): cu = this%cu + cu (likewise dcu, amu),
+// this%cu *= dex, and after the last species cu, amu, dcu *= dex.  Of the
+// species' own planes only this%cu(3) is kept (its one later reader, cbq).
+template <typename R>
+static int amjdp_fused_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu,
+                         qp_field2d *amu, double dex, bool first, bool last)
+{
+    const size_t pl = sp->ctx->rplane;
+    R *c = cu->rs.as<R>(), *d = dcu->rs.as<R>(), *a = amu->rs.as<R>();
+    R *planes[8] = {c, c + pl, c + 2 * pl, d, d + pl, a, a + pl, a + 2 * pl};
+    DepOut<R> o{};
+    for (int k = 0; k < 8; ++k) {
+        o.p[k] = planes[k];
+        o.inner[k] = first ? nullptr : planes[k];
+    }
+    o.p2[2] = sp->cu->rs.as<R>() + 2 * pl;
+    o.scale = last ? (R)dex : R(1);
+    o.scale2 = (R)dex;
+    int rc = amjdp_out<R>(sp, ef, bf, psit, dex, o);
+    sp->cu->state = 0;
+    cu->state = dcu->state = amu->state = 0;
+    return rc;
+}
+
+int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                        const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
+                        double dex, bool first, bool last)
+{
+    return DISPATCH(sp->ctx, amjdp_fused_t, sp, ef, bf, psit, cu, dcu, amu, dex, first, last);
+}
 }  // namespace qp

@@ -14,6 +14,10 @@ namespace qp {
 int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                          const qp_field2d *psit, double dex);
 int species_check(qp_species *sp);
+int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first);
+int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                        const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
+                        double dex, bool first, bool last);
 }  // namespace qp
 
 struct qp_sim {
@@ -109,33 +113,20 @@ int qp_sim_begin_step(qp_sim *s)
     return QP_OK;
 }
 
-// bxyz(1:2) = (-dex*epw(2), dex*epw(1)); bxyz(3) *= dex; bxyz(1) -= epwb(2);
-// bxyz(2) += epwb(1); all in k space (simulation_class.f03:677-680)
-static int assemble_b(qp_sim *s, double dex)
-{
-    qp_field2d *bxyz = s->f[QP_F_BXYZ], *epw = s->f[QP_F_EPW], *epwb = s->f[QP_F_EPWB];
-    const int d12[2] = {1, 2}, d21[2] = {2, 1}, d1[1] = {1}, d2[1] = {2}, d3[1] = {3};
-    const double v1[2] = {-dex, dex}, v3[1] = {dex};
-    RC(qp_field2d_mult(bxyz, epw, 2, d12, d21, v1));
-    RC(qp_field2d_mult(bxyz, bxyz, 1, d3, d3, v3));
-    RC(qp_field2d_sub(bxyz, bxyz, epwb, 1, d1, d1, d2));
-    RC(qp_field2d_add(bxyz, bxyz, epwb, 1, d2, d2, d1));
-    return QP_OK;
-}
-
-// cu -> div -> pot -> psit * (-dex) -> real; cu -> bxyz (k)  (:705-709)
-static int psit_and_b(qp_sim *s, double dex)
-{
-    qp_field2d *cu = s->f[QP_F_CU], *dv = s->f[QP_F_DIVVPOT], *psit = s->f[QP_F_PSIT];
-    const double m = -dex;
-    RC(qp_field2d_div(cu, dv));
-    RC(qp_field2d_pot(dv, psit));
-    RC(qp_field2d_mult(psit, psit, 0, nullptr, nullptr, &m));
-    RC(qp_field2d_fftkr(psit, 1));
-    RC(qp_field2d_bf(cu, s->f[QP_F_BXYZ]));
-    return QP_OK;
-}
-
+// The slice follows go_simulation (simulation_class.f03:643-736) call for
+// call, with these fusions (each kernel restates the calls it replaces in
+// their order):
+//  * the plasma deposits write the slice's q (qdp additions) and cu, dcu,
+//    amu (amjdp additions and the dex scaling of :682-685) directly;
+//  * pot + grad is one k-space kernel; the inverse transforms of fxy and
+//    psi, of psit and bxyz, and the forward transforms of cu, dcu, amu are
+//    batched;
+//  * a corrector pass (bfqp, then div, pot, * -dex for psit, then bf) and
+//    the B assembly that follows it (:677-680) are one k-space kernel, so the
+//    inverse transform of bxyz that opens the next pass (or follows the loop)
+//    runs at the end of this one, beside psit's;
+//  * -fxy + B, the predictor cu -= dxi dcu and the slice writes are one
+//    kernel each.
 int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, double *residual)
 {
     QP_REQUIRE(s, QP_ERR_ARG, "qp_slice_run: null sim");
@@ -147,68 +138,73 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     qp_field2d *qb = F[QP_F_QB], *qe = F[QP_F_QE], *psi = F[QP_F_PSI], *fxy = F[QP_F_FXY];
     qp_field2d *cu = F[QP_F_CU], *dcu = F[QP_F_DCU], *amu = F[QP_F_AMU], *bxyz = F[QP_F_BXYZ];
     qp_field2d *psit = F[QP_F_PSIT], *epw = F[QP_F_EPW], *epwb = F[QP_F_EPWB];
+    const int nsp = (int)s->sp.size();
     // beam charge and its transverse field (:650-656)
     RC(qp_field2d_as(qb, 0.0));
     for (qp_beam *b : s->bm) RC(qp_beam_qdpcopy(b, qb, j + 1));
     RC(qp_field2d_fftrk(qb, 1));
     RC(qp_field2d_elf(qb, epwb));
     // plasma charge, psi and E_perp (:657-670)
-    RC(qp_field2d_as(qe, 0.0));
-    for (qp_species *sp : s->sp) RC(qp_species_qdp(sp, qe));
+    if (nsp == 0) RC(qp_field2d_as(qe, 0.0));
+    for (int i = 0; i < nsp; ++i) RC(species_qdp_fused(s->sp[i], qe, i == 0));
     RC(qp_field2d_fftrk(qe, 1));
-    RC(qp_field2d_pot(qe, psi));
-    RC(qp_field2d_grad(psi, fxy));
-    RC(qp_field2d_fftkr(fxy, 1));
-    RC(qp_field2d_fftkr(psi, 1));
+    RC(fused_psi_fxy(qe, psi, fxy));
+    {
+        qp_field2d *fs[2] = {fxy, psi};
+        const int kinds[2] = {1, 1};
+        RC(fft_k2r_batch(c, fs, kinds, 2));
+        fxy->state = psi->state = 0;
+    }
     for (qp_species *sp : s->sp) RC(qp_species_extpsi(sp, psi, dex));
     const double dd = dex * dex;
     RC(qp_field2d_mult(psi, psi, 0, nullptr, nullptr, &dd));
-    // predictor fields from the carried current (:671-675)
-    RC(psit_and_b(s, dex));
+    // predictor fields from the carried current (:671-675) and the B assembly
+    const PassFields pf{cu, dcu, amu, epw, epwb, psit, bxyz};
+    qp_field2d *inv[2] = {psit, bxyz};
+    const int inv_kinds[2] = {1, 2};
+    qp_field2d *fwd[3] = {cu, dcu, amu};
+    const int fwd_kinds[3] = {1, 1, 3};
+    RC(fused_pass(pf, dex, false, nullptr));
+    RC(fft_k2r_batch(c, inv, inv_kinds, 2));
+    psit->state = bxyz->state = 0;
     // corrector passes (:676-710)
+    double *nd = c->red.as<double>() + 2 * BFQP_BLOCKS;
     int l = 0;
     double res = 0.0;
     const bool adaptive = tol > 0.0;
     for (l = 1; l <= iter_max; ++l) {
-        RC(assemble_b(s, dex));
-        RC(qp_field2d_fftkr(bxyz, 2));
-        RC(qp_field2d_as(cu, 0.0));
-        RC(qp_field2d_as(dcu, 0.0));
-        RC(qp_field2d_as(amu, 0.0));
-        for (qp_species *sp : s->sp) RC(qp_species_amjdp(sp, fxy, bxyz, psit, cu, amu, dcu, dex));
-        RC(qp_field2d_mult(cu, cu, 0, nullptr, nullptr, &dex));
-        RC(qp_field2d_mult(amu, amu, 0, nullptr, nullptr, &dex));
-        RC(qp_field2d_mult(dcu, dcu, 0, nullptr, nullptr, &dex));
-        RC(qp_field2d_fftrk(cu, 1));
-        RC(qp_field2d_fftrk(dcu, 1));
-        RC(qp_field2d_fftrk(amu, 3));
-        RC(bfqp_impl(cu, dcu, amu, epw, dd, dex, adaptive ? &res : nullptr));
-        RC(psit_and_b(s, dex));
-        if (adaptive && res < tol) break;
+        if (nsp == 0) {
+            RC(qp_field2d_as(cu, 0.0));
+            RC(qp_field2d_as(dcu, 0.0));
+            RC(qp_field2d_as(amu, 0.0));
+        }
+        for (int i = 0; i < nsp; ++i)
+            RC(species_amjdp_fused(s->sp[i], fxy, bxyz, psit, cu, dcu, amu, dex, i == 0,
+                                   i == nsp - 1));
+        RC(fft_r2k_batch(c, fwd, fwd_kinds, 3));
+        cu->state = dcu->state = amu->state = 1;
+        RC(fused_pass(pf, dex, true, nd));
+        RC(fft_k2r_batch(c, inv, inv_kinds, 2));
+        psit->state = bxyz->state = 0;
+        if (adaptive) {
+            double h[2];
+            QP_HIP(hipMemcpyAsync(h, nd, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+            QP_HIP(hipStreamSynchronize(c->stream));
+            res = h[1] > 0 ? std::sqrt(h[0] / h[1]) : (h[0] > 0 ? INFINITY : 0.0);
+            if (res < tol) break;
+        }
     }
     if (l > iter_max) l = iter_max;
     // species charge diagnostic of the last pass (cbq, :692-695): it reads
     // only the species' own q and cu, which the later steps leave untouched
     for (size_t i = 0; i < s->sp.size(); ++i) RC(qp_species_cbq(s->sp[i], s->q3s[i], j + 1));
-    // final B, E_perp, predictor current (:717-726)
-    RC(assemble_b(s, dex));
-    RC(qp_field2d_fftkr(bxyz, 2));
-    const double m1 = -1.0;
-    const int d1[1] = {1}, d2[1] = {2}, d12[2] = {1, 2}, d3[1] = {3}, d123[3] = {1, 2, 3};
-    RC(qp_field2d_mult(fxy, fxy, 0, nullptr, nullptr, &m1));
-    RC(qp_field2d_add(fxy, fxy, bxyz, 1, d1, d1, d2));
-    RC(qp_field2d_sub(fxy, fxy, bxyz, 1, d2, d2, d1));
-    RC(qp_field2d_mult(dcu, dcu, 0, nullptr, nullptr, &dxi));
-    RC(qp_field2d_sub(cu, cu, dcu, 2, d12, d12, d12));
+    // final E_perp and predictor current (:721-726); B is already assembled
+    RC(fused_efield(fxy, bxyz));
+    RC(fused_predict_cu(cu, dcu, dxi));
     // plasma push (:727-730)
     for (qp_species *sp : s->sp) RC(species_push_nocheck(sp, fxy, bxyz, psit, dex));
     // slice j+1 of the 3-D wake (:731-735)
-    RC(qp_field2d_mult(fxy, fxy, 0, nullptr, nullptr, &dex));
-    RC(qp_field2d_cb(fxy, s->bexyz, j + 1, 2, d12, d12));
-    RC(qp_field2d_cb(psit, s->bexyz, j + 1, 1, d1, d3));
-    const double vd[2] = {dex, dex};
-    RC(qp_field2d_mult(bxyz, bxyz, 2, d12, d12, vd));
-    RC(qp_field2d_cb(bxyz, s->bbxyz, j + 1, 3, d123, d123));
+    RC(fused_slice_write(fxy, psit, bxyz, s->bexyz, s->bbxyz, j + 1, dex));
     for (qp_species *sp : s->sp) RC(species_check(sp));
     if (iters_used) *iters_used = l;
     if (residual) *residual = adaptive ? res : -1.0;
