@@ -54,15 +54,14 @@ def run_gpu(args, precision, beam_part=None):
         f"(plasma {sp.np}, beam {bm.np})")
     sim.begin_step()
     # first timed slice: a little ahead of the beam centre (xi = 2)
-    j_first = max(1, int(round((2.0 - 2.0 * 0.5) / ctx.dxi)))
+    j_first = args.j_first or max(1, int(round((2.0 - 2.0 * 0.5) / ctx.dxi)))
     t0 = time.time()
     for j in range(1, j_first - args.warmup):
         sim.slice(j, iter_max=args.iter_max, tol=args.tol)
     log(f"[bench] fast-forward to slice {j_first - args.warmup}: {time.time() - t0:.1f}s")
     for j in range(max(1, j_first - args.warmup), j_first):
         sim.slice(j, iter_max=args.iter_max, tol=args.tol)
-    ctx.sync()
-    ctx.profile(True)
+    # timed region: no per-launch events
     passes = []
     ctx.sync()
     t0 = time.perf_counter()
@@ -71,8 +70,44 @@ def run_gpu(args, precision, beam_part=None):
         passes.append(it)
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    # per-kernel-class durations (HIP events around each launch on the
+    # library's stream) over the next slices, outside the timed region
+    nprof = min(args.prof_steps, ctx.nz - j_first - args.steps)
+    ctx.profile(True)
+    for j in range(j_first + args.steps, j_first + args.steps + nprof):
+        sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+    ctx.sync()
     prof = {qps.PROF_NAMES[k]: ctx.profile_get(k) for k in range(len(qps.PROF_NAMES))}
     ctx.profile(False)
+    sweep = {}
+    if args.sweep:
+        # the deposit launch shapes, each over the same profiled slices
+        j0 = j_first + args.steps + nprof
+        for knob, name in ((qps.TUNE_AMJDP, "amjdp"), (qps.TUNE_DEPOSIT_Q, "deposit_q")):
+            _, nv = ctx.tune_get(knob)
+            kind = qps.PROF_NAMES.index(name)
+            for v in range(nv):
+                ctx.tune(knob, v)
+                ctx.profile(True)
+                for j in range(j0, j0 + 3):
+                    sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+                t, cnt, _ = ctx.profile_get(kind)
+                ctx.profile(False)
+                sweep[f"{name}[{v}]"] = round(t / max(cnt, 1), 4)
+                j0 += 3
+            ctx.tune(knob, -1)
+        # corrector speculation off / on: wall time per slice
+        for v in (0, 1):
+            ctx.tune(qps.TUNE_SPECULATE, v)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for j in range(j0, j0 + 5):
+                sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+            ctx.sync()
+            sweep[f"speculate[{v}] ms/slice"] = round((time.perf_counter() - t0) / 5 * 1e3, 4)
+            j0 += 5
+        ctx.tune(qps.TUNE_SPECULATE, -1)
+        log(f"[bench] sweep (ms per launch): {sweep}")
     # every kernel class: time, launches and algorithmic bytes (operands read
     # or written once) counted by the library at each launch
     classes = {}
@@ -93,6 +128,7 @@ def run_gpu(args, precision, beam_part=None):
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(t / cnt, 4), "algorithmic_bytes_per_launch": b / cnt}
     total_ms = sum(v[0] for v in prof.values())
+    nprof = max(nprof, 1)
     result = {
         "slices_per_s": args.steps / elapsed,
         "ms_per_slice": elapsed / args.steps * 1e3,
@@ -101,13 +137,27 @@ def run_gpu(args, precision, beam_part=None):
         "first_slice": j_first,
         "roofline": roof,
         "kernels": classes,
-        "kernel_ms_per_slice": total_ms / args.steps,
+        "kernel_ms_per_slice": total_ms / nprof,
         "np_plasma": sp.np,
         "np_beam": bm.np,
+        "sweep": sweep,
     }
     part = bm.download() if beam_part is None and args.fp64 else None
     ctx.close()
     return result, part
+
+
+def calib_copy(args):
+    """A plain copy of known size (1 GiB read, 1 GiB written) for scaling the
+    HBM byte counters of a profiler run."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 3 x {n * 4} B read and written")
 
 
 def cpu_baseline(args):
@@ -147,11 +197,22 @@ def main():
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")
     ap.add_argument("--cpu-slices", type=int, default=2)
+    ap.add_argument("--prof-steps", type=int, default=10,
+                    help="slices after the timed ones run with per-launch events")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time every compiled deposit launch shape")
+    ap.add_argument("--j-first", type=int, default=0,
+                    help="first timed slice (default: ahead of the beam centre)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 1 GiB copy kernel (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        calib_copy(args)
+        return
     import qpslice as qps
     r32, beam = run_gpu(args, qps.FP32)
     log(f"[bench] FP32: {r32['slices_per_s']:.1f} slices/s, passes {r32['passes_mean']:.2f}")

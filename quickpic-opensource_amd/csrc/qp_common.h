@@ -29,6 +29,8 @@ constexpr int HALO = 4 * TILE;           // border nodes of a 17x17 tile block
 constexpr double TWOPI_REF = 6.28318530717959;  // the reference's 2*pi literal
 constexpr double BEAM_FIX = 17592186044416.0;   // 2^44: beam charge fixed point
 constexpr int MAX_BATCH = 8;            // components in one batched transform
+constexpr int RES_SLOTS = 64;           // pinned residual slots of the corrector loop
+constexpr int FLAG_WORDS = 64;          // converged flags (passes 0..63)
 
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
@@ -64,10 +66,26 @@ struct DevBuf {
 }  // namespace qp
 
 namespace qp {
+// The corrector's residual: the per-block FP64 partial sums of the corrector
+// kernel are added up in block order by one block of the transform launched
+// right after it (nblk = 0: nothing to do), which writes the residual
+// sqrt(num/den) to h_slot and num, den to out, and sets *flag_out when
+// *flag_in is set or the residual is below tol.
+struct ResJob {
+    const double *partial;
+    int nblk;
+    const int *flag_in;
+    int *flag_out;
+    double *h_slot;
+    double *out;
+    double tol;
+};
+
 struct ProfEv {
     hipEvent_t a, b;
     int kind;
     double bytes;
+    int tag;
 };
 }  // namespace qp
 
@@ -90,11 +108,22 @@ struct qp_ctx {
     size_t max_lds;               // LDS bytes one workgroup may use
     // kernel-class timing (qp_ctx_profile)
     int prof_on;
+    int prof_tag;         // tag of the launches being enqueued (prof_discard)
     std::vector<qp::ProfEv> prof_pending;
     std::vector<hipEvent_t> prof_pool;
     double prof_ms[QP_PROF_COUNT];
     int64_t prof_n[QP_PROF_COUNT];
     double prof_bytes[QP_PROF_COUNT];
+    // corrector-loop control.  flags (device ints): word l is set once pass l
+    // (or an earlier one) has converged; the kernels of pass l+1 test it and
+    // return at once.  h_res (pinned host): residual of pass l in slot
+    // l % RES_SLOTS.
+    qp::DevBuf flags;
+    double *h_res;
+    qp::ResJob res_job;   // residual sum carried by the next transform launch
+    hipEvent_t pass_ev[2];
+    const int *skip;      // non-null while a speculative pass is being enqueued
+    int tune[QP_TUNE_COUNT];   // launch-shape variants (qp_ctx_tune)
 };
 
 struct qp_field2d {
@@ -161,13 +190,44 @@ int bfqp_impl(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy,
 
 constexpr int BFQP_BLOCKS = 1024;   // blocks of the corrector kernels (FP64 partials)
 __global__ void k_sum_partials(const double *partial, int nblk, double *out);
+// ResJob, run by the first wave of one block: lane t adds partials t, t+64,
+// ... in order, then the 64 lane sums are combined by a fixed butterfly
+__device__ __forceinline__ void run_res_job(const ResJob &j)
+{
+    if (j.nblk <= 0 || threadIdx.x >= 64) return;
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int i = t; i < j.nblk; i += 64) {
+        a += j.partial[2 * i];
+        b += j.partial[2 * i + 1];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    if (t == 0) {
+        const double res = b > 0.0 ? sqrt(a / b) : (a > 0.0 ? INFINITY : 0.0);
+        j.out[0] = a;
+        j.out[1] = b;
+        if (j.flag_out)
+            *j.flag_out = ((j.flag_in && *j.flag_in) || (j.tol > 0.0 && res < j.tol)) ? 1 : 0;
+        if (j.h_slot) {
+            *j.h_slot = res;
+            __threadfence_system();
+        }
+    }
+}
 
 // ---- fused slice-loop kernels (qp_fused.hip) ----
 struct PassFields {
     qp_field2d *cu, *dcu, *amu, *epw, *epwb, *psit, *bxyz;
 };
 int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
-int fused_pass(const PassFields &f, double dex, bool qp, double *num_den_dev);
+// qp: one corrector pass; otherwise the predictor (which also clears the
+// converged flag).  With qp the residual of pass l is written to h_res and,
+// when tol > 0 and it is below tol, the converged flag is set.
+int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol);
 int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
 int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
 int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
@@ -176,6 +236,9 @@ int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
 // ---- kernel-class timing ----
 int prof_begin(qp_ctx *c, int kind, double bytes);   // returns a slot or -1
 void prof_end(qp_ctx *c, int slot);
+// forget the timing of pending launches with this tag (a speculative pass
+// whose kernels returned at once)
+void prof_discard(qp_ctx *c, int tag);
 
 // Brackets one launch with events while profiling is on; bytes is the
 // launch's algorithmic traffic.

@@ -127,7 +127,7 @@ static hipEvent_t pool_get(qp_ctx *c)
 
 int prof_begin(qp_ctx *c, int kind, double bytes)
 {
-    ProfEv ev{pool_get(c), pool_get(c), kind, bytes};
+    ProfEv ev{pool_get(c), pool_get(c), kind, bytes, c->prof_tag};
     if (!ev.a || !ev.b) return -1;
     if (hipEventRecord(ev.a, c->stream) != hipSuccess) return -1;
     c->prof_pending.push_back(ev);
@@ -137,6 +137,20 @@ int prof_begin(qp_ctx *c, int kind, double bytes)
 void prof_end(qp_ctx *c, int slot)
 {
     (void)hipEventRecord(c->prof_pending[slot].b, c->stream);
+}
+
+void prof_discard(qp_ctx *c, int tag)
+{
+    std::vector<ProfEv> keep;
+    for (ProfEv &e : c->prof_pending) {
+        if (e.tag == tag) {
+            c->prof_pool.push_back(e.a);
+            c->prof_pool.push_back(e.b);
+        } else {
+            keep.push_back(e);
+        }
+    }
+    c->prof_pending.swap(keep);
 }
 
 static int prof_resolve(qp_ctx *c)
@@ -179,6 +193,7 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     c->nx = 1 << indx; c->ny = 1 << indy; c->nz = 1 << indz;
     c->prec = precision;
     c->device = device;
+    c->tune[QP_TUNE_SPECULATE] = 1;
     c->esize = precision == QP_FP64 ? 8 : 4;
     std::memcpy(c->box, box, sizeof(c->box));
     // simulation_class.f03:174-188: dx = (max-min)/2^indx etc.
@@ -204,7 +219,15 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     if (!rc) rc = c->scr_k.alloc(MAX_BATCH * pl * c->esize);
     if (!rc) rc = c->halo.alloc((size_t)8 * c->ntiles * HALO * c->esize);
     if (!rc) rc = c->red.alloc(sizeof(double) * 4096);
+    if (!rc) rc = c->flags.alloc(sizeof(int) * 64);
     if (rc) { qp_ctx_destroy(c); return rc; }
+    e = hipHostMalloc((void **)&c->h_res, sizeof(double) * RES_SLOTS, hipHostMallocDefault);
+    if (e != hipSuccess) { c->h_res = nullptr; qp_ctx_destroy(c); return hip_fail(e, "hipHostMalloc", __FILE__, __LINE__); }
+    for (int i = 0; i < RES_SLOTS; ++i) c->h_res[i] = -1.0;
+    for (int i = 0; i < 2; ++i) {
+        e = hipEventCreateWithFlags(&c->pass_ev[i], hipEventDisableTiming);
+        if (e != hipSuccess) { c->pass_ev[i] = nullptr; qp_ctx_destroy(c); return hip_fail(e, "hipEventCreate", __FILE__, __LINE__); }
+    }
     *out = c;
     return QP_OK;
 }
@@ -216,6 +239,10 @@ int qp_ctx_destroy(qp_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->ffg.release(); c->ffs.release(); c->twx.release(); c->twy.release();
     c->scr_r.release(); c->scr_k.release(); c->halo.release(); c->red.release();
+    c->flags.release();
+    if (c->h_res) (void)hipHostFree(c->h_res);
+    for (int i = 0; i < 2; ++i)
+        if (c->pass_ev[i]) (void)hipEventDestroy(c->pass_ev[i]);
     for (ProfEv &e : c->prof_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -245,6 +272,27 @@ int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count, do
     if (total_ms) *total_ms = c->prof_ms[kind];
     if (count) *count = c->prof_n[kind];
     if (bytes) *bytes = c->prof_bytes[kind];
+    return QP_OK;
+}
+
+int qp_ctx_tune(qp_ctx *c, int knob, int value)
+{
+    static const int nvals[QP_TUNE_COUNT] = {4, 6, 2};
+    static const int dflt[QP_TUNE_COUNT] = {0, 0, 1};
+    QP_REQUIRE(c && knob >= 0 && knob < QP_TUNE_COUNT, QP_ERR_ARG, "qp_ctx_tune: bad knob");
+    if (value < 0) value = dflt[knob];
+    QP_REQUIRE(value < nvals[knob], QP_ERR_ARG, "qp_ctx_tune: knob %d has %d values", knob,
+               nvals[knob]);
+    c->tune[knob] = value;
+    return QP_OK;
+}
+
+int qp_ctx_tune_get(const qp_ctx *c, int knob, int *value, int *nvalues)
+{
+    static const int nvals[QP_TUNE_COUNT] = {4, 6, 2};
+    QP_REQUIRE(c && knob >= 0 && knob < QP_TUNE_COUNT, QP_ERR_ARG, "qp_ctx_tune_get: bad knob");
+    if (value) *value = c->tune[knob];
+    if (nvalues) *nvalues = nvals[knob];
     return QP_OK;
 }
 

@@ -93,9 +93,11 @@ template <typename R> struct CompSet {
 template <typename R, int L>
 __global__ void __launch_bounds__(FftShape<L>::BS)
 k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
-           const cplx<R> *__restrict__ W, int guard_row)
+           const cplx<R> *__restrict__ W, int guard_row, const int *skip, ResJob rj)
 {
     using S = FftShape<L>;
+    if (skip && *skip) return;
+    if (blockIdx.x == 0 && blockIdx.y == 0) run_res_job(rj);
     constexpr int M = S::M, n = S::N, TF = S::TF;
     extern __shared__ unsigned char smem_raw[];
     const int comp = blockIdx.y;
@@ -235,9 +237,10 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
 // dst(c)[col][row] = src(c)[row][col] for row < rows, col < cols
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_transpose(CompSet<R> cs, int ld_src, int ld_dst, int rows, int cols)
+k_transpose(CompSet<R> cs, int ld_src, int ld_dst, int rows, int cols, const int *skip)
 {
     __shared__ R tile[32][33];
+    if (skip && *skip) return;
     const int comp = blockIdx.z;
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -288,8 +291,10 @@ static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, in
     dim3 grid((pairs + S::FB - 1) / S::FB, ncomp);
     {
     ProfScope _ps(c, QP_PROF_FFT, 2.0 * ncomp * nrows * (S::N + 1.0) * sizeof(R));
+    const ResJob rj = c->res_job;
+    c->res_job = ResJob{};
     hipLaunchKernelGGL(kern, grid, dim3(S::BS), lds, c->stream, cs, nrows, ld_src, ld_dst, types,
-                       (R)scale, W, guard_row);
+                       (R)scale, W, guard_row, c->skip, rj);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -322,7 +327,7 @@ static int transpose(qp_ctx *c, const CompSet<R> &cs, int ncomp, int ld_src, int
     {
     ProfScope _ps(c, QP_PROF_TRANSPOSE, 2.0 * ncomp * (double)rows * cols * sizeof(R));
     hipLaunchKernelGGL(k_transpose<R>, grid, dim3(256), 0, c->stream, cs, ld_src, ld_dst, rows,
-                       cols);
+                       cols, c->skip);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;

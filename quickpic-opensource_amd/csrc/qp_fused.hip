@@ -41,9 +41,11 @@ template <typename R, bool QP>
 __global__ void __launch_bounds__(256)
 k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *__restrict__ cu,
        R *dcu, const R *__restrict__ amu, R *epw, const R *__restrict__ epwb, R *psit, R *bxyz,
-       R aa, R idex, R dex, double *partial)
+       R aa, R idex, R dex, double *partial, int *flags, const int *skip)
 {
     __shared__ double s_num[256], s_den[256];
+    if (skip && *skip) return;
+    if (!QP && blockIdx.x == 0 && threadIdx.x < FLAG_WORDS) flags[threadIdx.x] = 0;
     double num = 0.0, den = 0.0;
     const size_t cs = g.cs;
     const int total = (g.nx + 1) * (g.ny + 1);
@@ -204,7 +206,7 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy)
 }
 
 template <typename R>
-static int pass_t(const PassFields &f, double dex, bool qp, double *num_den_dev)
+static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol)
 {
     qp_ctx *c = f.cu->ctx;
     double *part = c->red.as<double>();
@@ -217,23 +219,25 @@ static int pass_t(const PassFields &f, double dex, bool qp, double *num_den_dev)
                        c->ffg.as<R>(), c->ffs.as<R>(), f.cu->ks.as<R>(), f.dcu->ks.as<R>(),
                        f.amu->ks.as<R>(), f.epw->ks.as<R>(), f.epwb->ks.as<R>(),
                        f.psit->ks.as<R>(), f.bxyz->ks.as<R>(), (R)(dex * dex), (R)(1.0 / dex),
-                       (R)dex, part);
+                       (R)dex, part, c->flags.as<int>(), qp ? c->skip : nullptr);
     }
     QP_LAUNCH_CHECK();
     if (qp) {
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, part, BFQP_BLOCKS,
-                           num_den_dev);
-        QP_LAUNCH_CHECK();
+        // the next transform launch sums the partials (see ResJob)
+        int *fl = c->flags.as<int>();
+        c->res_job = ResJob{part, BFQP_BLOCKS, l > 0 && l < FLAG_WORDS ? fl + l - 1 : nullptr,
+                            l > 0 && l < FLAG_WORDS ? fl + l : nullptr,
+                            c->h_res + (l % RES_SLOTS), part + 2 * BFQP_BLOCKS, tol};
         f.epw->state = f.dcu->state = 1;
     }
     f.psit->state = f.bxyz->state = 1;
     return QP_OK;
 }
 
-int fused_pass(const PassFields &f, double dex, bool qp, double *num_den_dev)
+int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol)
 {
-    return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, num_den_dev)
-                                      : pass_t<float>(f, dex, qp, num_den_dev);
+    return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, l, tol)
+                                      : pass_t<float>(f, dex, qp, l, tol);
 }
 
 template <typename R>

@@ -30,6 +30,10 @@ struct qp_species {
     qp::DevBuf scan_tmp, err;
     size_t scan_bytes;
     qp_field2d *q, *qn, *cu, *dcu, *amu;
+    // deferred move check of the slice loop: pinned copy of err and its event
+    int *h_err;
+    hipEvent_t err_ev;
+    bool err_pending;
 };
 
 namespace qp {
@@ -127,8 +131,9 @@ template <typename R> static DepOut<R> dep_plain(R *const *planes, int n)
 // with fold, row ny is added into row 0 and cleared (PPNACGUARD2L nvp = 1).
 template <typename R>
 __global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t halo_cs,
-                                 DepOut<R> out, int fold)
+                                 DepOut<R> out, int fold, const int *skip)
 {
+    if (skip && *skip) return;
     const int comp = blockIdx.y;
     const int rows_n = (g.my1 + 1) * (g.nx + 1);
     const int cols_n = (g.mx1 + 1) * g.my1 * (TILE - 1);
@@ -173,6 +178,11 @@ __global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t 
 // per-cell strided reads over the banks
 __device__ __forceinline__ int lpad(int p) { return p + (p >> 5); }
 
+// 1/x: IEEE division in FP64 (the reference's arithmetic); the hardware
+// reciprocal (1 ulp) in FP32, whose results are compared with FP64 at 1e-5
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <int CH> struct Stage {
     static constexpr int N = CH + CH / 32 + 1;
 };
@@ -203,8 +213,8 @@ __device__ __forceinline__ void emit_nodes(const R *nacc, const DepOut<R> &out, 
 // atomic touches the deposit.
 
 // PPGPPOST2L (part2d_lib77.f:194-309): q(n,m) += qm (1-dx)(1-dy) etc.
-template <typename R, int RG, int CH>
-__global__ void __launch_bounds__(256)
+template <typename R, int RG, int CH, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R *halo)
 {
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
@@ -287,11 +297,13 @@ k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R d
 
 // PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768): half Boris step with the B_z
 // rotation, deposit of J (3), dJ/dxi (2) and momentum flux (3); p6 updated.
-template <typename R, int RG, int CH>
-__global__ void __launch_bounds__(256)
+template <typename R, int RG, int CH, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
-        const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex)
+        const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
+        const int *skip)
 {
+    if (skip && *skip) return;
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
     extern __shared__ unsigned char smem_raw[];
     R *sx = reinterpret_cast<R *>(smem_raw), *sy = sx + NP, *spx = sy + NP, *spy = spx + NP;
@@ -341,7 +353,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
                 R p6 = sp6[l], p7 = sp7[l], qm = sq[l];
                 R dxp = sx[l] - fx, dyp = sy[l] - fy;
                 R amx = R(1) - dxp, amy = R(1) - dyp;
-                R inv_part_7 = R(1) / p7;
+                R inv_part_7 = recip(p7);
                 R qtmh1 = qtmh * inv_part_7;
                 R qtmh2 = qtmh1 * p6;
                 R dx = gather(ex, amx, amy, dxp, dyp);
@@ -356,7 +368,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
                 R acx = vx + ddx, acy = vy + ddy;
                 R omzt = qtmh1 * oz;
                 R omt = omzt * omzt;
-                R anorm = R(2) / (R(1) + omt);
+                R anorm = R(2) * recip(R(1) + omt);
                 R rot1 = R(0.5) * (R(1) - omt);
                 R rot2 = omzt;
                 R v1 = (rot1 * acx + rot2 * acy) * anorm + ddx;
@@ -647,23 +659,31 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
     {
     ProfScope _ps(c, QP_PROF_BORDER, 2.0 * ncomp * (double)n * sizeof(R));
     hipLaunchKernelGGL(k_deposit_border<R>, dim3((n + 255) / 256, ncomp), dim3(256), 0, c->stream,
-                       g, c->halo.as<R>(), (size_t)c->ntiles * HALO, out, fold);
+                       g, c->halo.as<R>(), (size_t)c->ntiles * HALO, out, fold, c->skip);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
 }
 
 // species charge into a real-space plane (overwritten), guard-folded
-// deposit tiling per precision: rows per group and staging chunk
-template <typename R> struct DepCfg;
-template <> struct DepCfg<float> {
-    static constexpr int Q_RG = 16, Q_CH = 4096;   // charge: one thread per cell
-    static constexpr int J_RG = 8, J_CH = 2048;    // current: two threads per cell
+// Deposit launch shapes: rows per group RG (16/RG lanes per cell), staging
+// chunk CH and a waves-per-SIMD floor WPE (4 caps the kernel at 128 VGPRs).
+// Variant 0 of each table is the default; qp_ctx_tune selects another.
+struct DepShape {
+    int rg, ch, wpe;
 };
-template <> struct DepCfg<double> {
-    static constexpr int Q_RG = 8, Q_CH = 2048;
-    static constexpr int J_RG = 4, J_CH = 1024;
+template <typename R> struct DepTab;
+template <> struct DepTab<float> {
+    static constexpr DepShape q[] = {{16, 2048, 1}, {16, 4096, 1}, {8, 2048, 1}, {16, 1024, 1}};
+    static constexpr DepShape j[] = {{8, 1024, 1}, {8, 2048, 1}, {4, 1024, 4}, {16, 1024, 4},
+                                     {4, 512, 4}, {8, 512, 1}};
 };
+template <> struct DepTab<double> {
+    static constexpr DepShape q[] = {{8, 1024, 1}, {8, 2048, 1}, {16, 1024, 1}, {8, 512, 1}};
+    static constexpr DepShape j[] = {{8, 1024, 1}, {4, 1024, 1}, {4, 512, 1}, {8, 512, 1},
+                                     {2, 512, 1}, {16, 512, 1}};
+};
+constexpr int NQVAR = 4, NJVAR = 6;
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
 template <typename K>
@@ -677,13 +697,13 @@ static int allow_lds(K kernel, size_t bytes, qp_ctx *c)
     return QP_OK;
 }
 
-template <typename R>
-static int deposit_q_out(qp_species *sp, const DepOut<R> &out)
+template <typename R, int V>
+static int deposit_q_v(qp_species *sp, const DepOut<R> &out)
 {
     qp_ctx *c = sp->ctx;
-    constexpr int RG = DepCfg<R>::Q_RG, CH = DepCfg<R>::Q_CH;
-    const size_t lds = (3 * (size_t)Stage<CH>::N + 289) * sizeof(R);
-    auto kern = k_deposit_q<R, RG, CH>;
+    constexpr DepShape d = DepTab<R>::q[V];
+    const size_t lds = (3 * (size_t)Stage<d.ch>::N + 289) * sizeof(R);
+    auto kern = k_deposit_q<R, d.rg, d.ch, d.wpe>;
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
     {
@@ -692,7 +712,21 @@ static int deposit_q_out(qp_species *sp, const DepOut<R> &out)
                        soa<R>(sp->a), sp->off.as<int>(), out, c->halo.as<R>());
     }
     QP_LAUNCH_CHECK();
-    return deposit_border<R>(c, out, 1, 1);
+    return QP_OK;
+}
+
+template <typename R>
+static int deposit_q_out(qp_species *sp, const DepOut<R> &out)
+{
+    int rc;
+    switch (sp->ctx->tune[QP_TUNE_DEPOSIT_Q]) {
+    case 1: rc = deposit_q_v<R, 1>(sp, out); break;
+    case 2: rc = deposit_q_v<R, 2>(sp, out); break;
+    case 3: rc = deposit_q_v<R, 3>(sp, out); break;
+    default: rc = deposit_q_v<R, 0>(sp, out); break;
+    }
+    if (rc) return rc;
+    return deposit_border<R>(sp->ctx, out, 1, 1);
 }
 
 template <typename R>
@@ -761,6 +795,9 @@ static int alloc_particles(qp_species *sp, int64_t np)
     if (!rc) rc = sp->off.alloc((size_t)(c->ncells + 1) * 4);
     if (!rc) rc = sp->err.alloc(sizeof(int));
     if (rc) return rc;
+    QP_HIP(hipHostMalloc((void **)&sp->h_err, sizeof(int), hipHostMallocDefault));
+    *sp->h_err = 0;
+    QP_HIP(hipEventCreateWithFlags(&sp->err_ev, hipEventDisableTiming));
     sp->scan_bytes = 0;
     QP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sp->scan_bytes, sp->count.as<int>(),
                                             sp->off.as<int>(), c->ncells + 1, c->stream));
@@ -781,16 +818,16 @@ static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
     return QP_OK;
 }
 
-template <typename R>
-static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
-                     const qp_field2d *psit, double dex, const DepOut<R> &out)
+template <typename R, int V>
+static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                   const qp_field2d *psit, double dex, const DepOut<R> &out)
 {
     qp_ctx *c = sp->ctx;
     const size_t pl = c->rplane;
     const size_t hcs = (size_t)c->ntiles * HALO;
-    constexpr int RG = DepCfg<R>::J_RG, CH = DepCfg<R>::J_CH;
-    const size_t lds = (7 * (size_t)Stage<CH>::N + 8 * 289) * sizeof(R);
-    auto kern = k_amjdp<R, RG, CH>;
+    constexpr DepShape d = DepTab<R>::j[V];
+    const size_t lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
+    auto kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
     {
@@ -798,10 +835,27 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
                        psit->rs.as<R>(), pl, out, c->halo.as<R>(), hcs, (R)sp->qbm, (R)sp->dt,
-                       (R)dex);
+                       (R)dex, c->skip);
     }
     QP_LAUNCH_CHECK();
-    return deposit_border<R>(c, out, 8, 1);
+    return QP_OK;
+}
+
+template <typename R>
+static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
+                     const qp_field2d *psit, double dex, const DepOut<R> &out)
+{
+    int rc;
+    switch (sp->ctx->tune[QP_TUNE_AMJDP]) {
+    case 1: rc = amjdp_v<R, 1>(sp, ef, bf, psit, dex, out); break;
+    case 2: rc = amjdp_v<R, 2>(sp, ef, bf, psit, dex, out); break;
+    case 3: rc = amjdp_v<R, 3>(sp, ef, bf, psit, dex, out); break;
+    case 4: rc = amjdp_v<R, 4>(sp, ef, bf, psit, dex, out); break;
+    case 5: rc = amjdp_v<R, 5>(sp, ef, bf, psit, dex, out); break;
+    default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out); break;
+    }
+    if (rc) return rc;
+    return deposit_border<R>(sp->ctx, out, 8, 1);
 }
 
 // the species' own cu, dcu, amu (overwritten)
@@ -883,6 +937,8 @@ int qp_species_destroy(qp_species *sp)
     for (int k = 0; k < 8; ++k) { sp->a[k].release(); sp->b[k].release(); }
     sp->key.release(); sp->slot.release(); sp->count.release(); sp->off.release();
     sp->scan_tmp.release(); sp->err.release();
+    if (sp->h_err) (void)hipHostFree(sp->h_err);
+    if (sp->err_ev) (void)hipEventDestroy(sp->err_ev);
     qp_field2d_destroy(sp->q); qp_field2d_destroy(sp->qn); qp_field2d_destroy(sp->cu);
     qp_field2d_destroy(sp->dcu); qp_field2d_destroy(sp->amu);
     delete sp;
@@ -1069,6 +1125,33 @@ int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d 
     return DISPATCH(sp->ctx, push_t, sp, ef, bf, psit, dex, false);
 }
 int species_check(qp_species *sp) { return check_err(sp, "pmove"); }
+
+// The slice loop does not wait for the move check of each push: the error
+// word is copied to pinned memory behind the push, and a later slice (or the
+// end of the step, which waits) reports it.
+int species_check_post(qp_species *sp)
+{
+    qp_ctx *c = sp->ctx;
+    QP_HIP(hipMemcpyAsync(sp->h_err, sp->err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    QP_HIP(hipEventRecord(sp->err_ev, c->stream));
+    sp->err_pending = true;
+    return QP_OK;
+}
+
+int species_check_poll(qp_species *sp, bool wait)
+{
+    if (!sp->err_pending) return QP_OK;
+    if (wait) QP_HIP(hipEventSynchronize(sp->err_ev));
+    else if (hipEventQuery(sp->err_ev) != hipSuccess) return QP_OK;
+    sp->err_pending = false;
+    const int h = *sp->h_err;
+    if (!h) return QP_OK;
+    *sp->h_err = 0;
+    QP_HIP(hipMemsetAsync(sp->err.p, 0, sizeof(int), sp->ctx->stream));
+    if (h >= (1 << 30)) set_error("pmove: particle position outside the grid");
+    else set_error("pmove: particle left its tile by more than one tile (tile %d)", h);
+    return QP_ERR_CHECK;
+}
 
 // qdp_species2d with the slice's additions fused into the deposit
 // (species2d_class.f03:166-185): this%q = deposit; q = this%q + q;

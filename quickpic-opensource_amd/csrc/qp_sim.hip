@@ -14,6 +14,8 @@ namespace qp {
 int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                          const qp_field2d *psit, double dex);
 int species_check(qp_species *sp);
+int species_check_post(qp_species *sp);
+int species_check_poll(qp_species *sp, bool wait);
 int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first);
 int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
@@ -139,6 +141,8 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     qp_field2d *cu = F[QP_F_CU], *dcu = F[QP_F_DCU], *amu = F[QP_F_AMU], *bxyz = F[QP_F_BXYZ];
     qp_field2d *psit = F[QP_F_PSIT], *epw = F[QP_F_EPW], *epwb = F[QP_F_EPWB];
     const int nsp = (int)s->sp.size();
+    // the move check of an earlier slice, if it has arrived
+    for (qp_species *sp : s->sp) RC(species_check_poll(sp, false));
     // beam charge and its transverse field (:650-656)
     RC(qp_field2d_as(qb, 0.0));
     for (qp_beam *b : s->bm) RC(qp_beam_qdpcopy(b, qb, j + 1));
@@ -164,15 +168,14 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     const int inv_kinds[2] = {1, 2};
     qp_field2d *fwd[3] = {cu, dcu, amu};
     const int fwd_kinds[3] = {1, 1, 3};
-    RC(fused_pass(pf, dex, false, nullptr));
+    RC(fused_pass(pf, dex, false, 0, 0.0));
     RC(fft_k2r_batch(c, inv, inv_kinds, 2));
     psit->state = bxyz->state = 0;
     // corrector passes (:676-710)
-    double *nd = c->red.as<double>() + 2 * BFQP_BLOCKS;
-    int l = 0;
-    double res = 0.0;
-    const bool adaptive = tol > 0.0;
-    for (l = 1; l <= iter_max; ++l) {
+    int *flags = c->flags.as<int>();
+    auto pass = [&](int l) -> int {
+        c->prof_tag = l;
+        if (c->skip) c->skip = flags + (l - 1);
         if (nsp == 0) {
             RC(qp_field2d_as(cu, 0.0));
             RC(qp_field2d_as(dcu, 0.0));
@@ -183,16 +186,48 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
                                    i == nsp - 1));
         RC(fft_r2k_batch(c, fwd, fwd_kinds, 3));
         cu->state = dcu->state = amu->state = 1;
-        RC(fused_pass(pf, dex, true, nd));
+        RC(fused_pass(pf, dex, true, l, tol));
         RC(fft_k2r_batch(c, inv, inv_kinds, 2));
         psit->state = bxyz->state = 0;
-        if (adaptive) {
-            double h[2];
-            QP_HIP(hipMemcpyAsync(h, nd, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        c->prof_tag = 0;
+        return QP_OK;
+    };
+    int l = 0;
+    double res = -1.0;
+    const bool adaptive = tol > 0.0;
+    if (!adaptive) {
+        for (l = 1; l <= iter_max; ++l) RC(pass(l));
+        l = iter_max;
+    } else if (nsp == 0 || !c->tune[QP_TUNE_SPECULATE] || iter_max >= FLAG_WORDS) {
+        // read each residual in turn
+        for (l = 1;; ++l) {
+            RC(pass(l));
             QP_HIP(hipStreamSynchronize(c->stream));
-            res = h[1] > 0 ? std::sqrt(h[0] / h[1]) : (h[0] > 0 ? INFINITY : 0.0);
-            if (res < tol) break;
+            res = c->h_res[l % RES_SLOTS];
+            if (res < tol || l == iter_max) break;
         }
+    } else {
+        // Pass l+1 is enqueued before the residual of pass l is read back, so
+        // the device never idles on the host; once pass l converges, flag
+        // word l is set and the kernels of the pass queued behind it return
+        // at once, leaving the fields of the converged pass.
+        struct SkipGuard {
+            qp_ctx *c;
+            ~SkipGuard() { c->skip = nullptr; }
+        } guard{c};
+        c->skip = flags;
+        RC(pass(1));
+        QP_HIP(hipEventRecord(c->pass_ev[1], c->stream));
+        for (l = 1;; ++l) {
+            if (l < iter_max) {
+                RC(pass(l + 1));
+                QP_HIP(hipEventRecord(c->pass_ev[(l + 1) & 1], c->stream));
+            }
+            QP_HIP(hipEventSynchronize(c->pass_ev[l & 1]));
+            res = c->h_res[l % RES_SLOTS];
+            if (res < tol || l == iter_max) break;
+        }
+        if (l < iter_max) prof_discard(c, l + 1);
     }
     if (l > iter_max) l = iter_max;
     // species charge diagnostic of the last pass (cbq, :692-695): it reads
@@ -205,7 +240,7 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     for (qp_species *sp : s->sp) RC(species_push_nocheck(sp, fxy, bxyz, psit, dex));
     // slice j+1 of the 3-D wake (:731-735)
     RC(fused_slice_write(fxy, psit, bxyz, s->bexyz, s->bbxyz, j + 1, dex));
-    for (qp_species *sp : s->sp) RC(species_check(sp));
+    for (qp_species *sp : s->sp) RC(species_check_post(sp));
     if (iters_used) *iters_used = l;
     if (residual) *residual = adaptive ? res : -1.0;
     return QP_OK;
@@ -215,6 +250,7 @@ int qp_sim_end_step(qp_sim *s, double snew)
 {
     QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_end_step: null sim");
     qp_ctx *c = s->ctx;
+    for (qp_species *sp : s->sp) RC(species_check_poll(sp, true));
     for (qp_beam *b : s->bm) RC(qp_beam_push(b, s->bexyz, s->bbxyz, c->dex, c->dxi));
     for (qp_species *sp : s->sp) RC(qp_species_renew(sp, snew));
     QP_HIP(hipStreamSynchronize(c->stream));

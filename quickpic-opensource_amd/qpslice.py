@@ -34,6 +34,7 @@ FP64, FP32 = 0, 1
  PROF_AMJDP, PROF_PUSH, PROF_SORT, PROF_BORDER, PROF_BEAM_DEPOSIT, PROF_BEAM_PUSH) = range(12)
 PROF_NAMES = ("fft", "transpose", "kspace", "elementwise", "deposit_q", "extpsi", "amjdp",
               "push", "sort", "border", "beam_deposit", "beam_push")
+TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE = 0, 1, 2
 (F_QB, F_QE, F_PSIT, F_PSI, F_DIVVPOT, F_FXY, F_BXYZ, F_CU, F_DCU, F_AMU,
  F_EPW, F_EPWB) = range(12)
 
@@ -126,6 +127,17 @@ class Ctx:
         ms, n, b = C.c_double(), C.c_int64(), C.c_double()
         _call(lib().qp_ctx_profile_get, self.h, kind, C.byref(ms), C.byref(n), C.byref(b))
         return ms.value, n.value, b.value
+
+    def tune(self, knob, value):
+        """Selects launch shape `value` of a deposit kernel (TUNE_*); -1 is
+        the default.  Shapes change only the order of per-cell sums."""
+        _call(lib().qp_ctx_tune, self.h, knob, value)
+
+    def tune_get(self, knob):
+        """(current value, number of values) of a tuning knob."""
+        v, n = C.c_int(), C.c_int()
+        _call(lib().qp_ctx_tune_get, self.h, knob, C.byref(v), C.byref(n))
+        return v.value, n.value
 
     def close(self):
         """Destroys the context after every object created on it."""
