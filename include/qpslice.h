@@ -99,9 +99,9 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * QP_TUNE_DEPOSIT_Q / QP_TUNE_AMJDP select one of the compiled shapes of the
  * charge / current deposit (rows per group, staging chunk, occupancy); shapes
  * differ only in how a cell's particles are split over lanes, so results
- * agree to rounding.  QP_TUNE_SPECULATE (default 1) lets qp_slice_run enqueue
- * corrector pass l+1 before reading the residual of pass l (0: wait for each
- * residual).  value -1 restores a knob's default. */
+ * agree to rounding.  QP_TUNE_SPECULATE = 1 lets qp_slice_run enqueue
+ * corrector pass l+1 before reading the residual of pass l (default 0: wait
+ * for each residual).  value -1 restores a knob's default. */
 enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_COUNT };
 int qp_ctx_tune(qp_ctx *ctx, int knob, int value);
 /* current value and the number of values of a knob */

@@ -193,7 +193,6 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     c->nx = 1 << indx; c->ny = 1 << indy; c->nz = 1 << indz;
     c->prec = precision;
     c->device = device;
-    c->tune[QP_TUNE_SPECULATE] = 1;
     c->esize = precision == QP_FP64 ? 8 : 4;
     std::memcpy(c->box, box, sizeof(c->box));
     // simulation_class.f03:174-188: dx = (max-min)/2^indx etc.
@@ -278,7 +277,7 @@ int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count, do
 int qp_ctx_tune(qp_ctx *c, int knob, int value)
 {
     static const int nvals[QP_TUNE_COUNT] = {4, 6, 2};
-    static const int dflt[QP_TUNE_COUNT] = {0, 0, 1};
+    static const int dflt[QP_TUNE_COUNT] = {0, 0, 0};
     QP_REQUIRE(c && knob >= 0 && knob < QP_TUNE_COUNT, QP_ERR_ARG, "qp_ctx_tune: bad knob");
     if (value < 0) value = dflt[knob];
     QP_REQUIRE(value < nvals[knob], QP_ERR_ARG, "qp_ctx_tune: knob %d has %d values", knob,

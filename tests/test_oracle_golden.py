@@ -1,0 +1,129 @@
+"""Pins the C oracle against the reference's own outputs stored in
+tests/golden/ref_vectors.npz (written by tests/golden/make_golden.py from the
+reference's F77 kernels compiled from source).  Runs without the reference
+tree.
+
+Bars as in test_oracle_vs_ref.py: per-particle kernels bit-exact, k-space
+operators and deposits to a few ulp, transforms to 1e-13 (an independent
+restatement of the same mathematics), ranorm and the beam loader bit-exact,
+beam push survivors identical and in the same order.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import qpo
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def g():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+WRAPPERS = [(1, 1), (1, 2), (1, 3), (2, 1), (2, 2), (2, 3), (3, 3)]
+
+
+@pytest.mark.parametrize("kind,dim", WRAPPERS)
+def test_transforms(g, kind, dim):
+    assert rel(qpo.r2k(kind, g[f"r2k_{kind}_{dim}_in"]), g[f"r2k_{kind}_{dim}_out"]) < 1e-13
+    assert rel(qpo.k2r(kind, g[f"k2r_{kind}_{dim}_in"]), g[f"k2r_{kind}_{dim}_out"]) < 1e-13
+
+
+def test_pois_table_and_kspace(g):
+    nx, ny = int(g["nx"]), int(g["ny"])
+    tab = qpo.pois_table(nx, ny)
+    assert np.array_equal(tab[0], g["pois_g"]) and np.array_equal(tab[1], g["pois_s"])
+    q = g["kop_q"]
+    assert rel(qpo.pot(tab, q), g["kop_pot"]) < 1e-15
+    assert rel(qpo.smooth(tab, q), g["kop_smooth"]) < 1e-15
+    assert rel(qpo.elf(tab, q), g["kop_elf"]) < 1e-15
+    assert rel(qpo.bf(tab, g["kop_cu"]), g["kop_bf"]) < 1e-15
+    assert rel(qpo.div(g["kop_f2"]), g["kop_div"]) < 1e-15
+    assert rel(qpo.grad(q), g["kop_grad"]) < 1e-15
+
+
+def test_bfqp(g):
+    nx, ny, dex = int(g["nx"]), int(g["ny"]), float(g["dex"])
+    tab = qpo.pois_table(nx, ny)
+    cu, dcu, amu, b = (g[f"bfqp_in_{i}"].copy() for i in range(4))
+    qpo.bfqp(tab, cu, dcu, amu, b, dex * dex, dex)
+    assert rel(b, g["bfqp_b"]) < 1e-15
+    assert rel(dcu, g["bfqp_dcu"]) < 1e-15
+
+
+def test_plasma_loader_matches_the_stored_input(g):
+    # dist2d_000 (fdist2d_class.f03:212-316): the stored input was made by
+    # the oracle's loader; its positions are the quiet-start lattice of Q1
+    nx, ny = int(g["nx"]), int(g["ny"])
+    part = qpo.plasma_init(nx, ny, 2, 2)
+    qpo.order_by_tile(nx, ny, part)
+    assert np.array_equal(part[:, [0, 1, 5, 6, 7]], g["plasma"][:, [0, 1, 5, 6, 7]])
+    assert part.shape[0] == (nx - 2) * (ny - 1) * 4
+
+
+def test_charge_deposit(g):
+    nx, ny = int(g["nx"]), int(g["ny"])
+    q = np.zeros((1, ny + 1, nx + 1))
+    qpo.qdeposit(nx, ny, g["plasma"].copy(), q)
+    qpo.guard_fold(q)
+    assert rel(q, g["qdeposit"]) < 1e-14
+
+
+def test_extpsi_amjdp(g):
+    dex, dxi = float(g["dex"]), float(g["dxi"])
+    part = g["plasma"].copy()
+    qpo.extpsi(part, g["extpsi_psi"], -1.0, dex)
+    assert np.array_equal(part, g["extpsi_out"])
+    fxy, bxy, psit = (g[f"amjdp_f_{i}"] for i in range(3))
+    ny1, nx1 = fxy.shape[1:]
+    cu, dcu, amu = (np.zeros((d, ny1, nx1)) for d in (3, 2, 3))
+    qpo.amjdp(part, fxy, bxy, psit, cu, dcu, amu, -1.0, dxi, dex)
+    assert np.array_equal(part, g["amjdp_part"])   # p6 updated in place
+    assert rel(cu, g["amjdp_cu"]) < 1e-13
+    assert rel(dcu, g["amjdp_dcu"]) < 1e-13
+    assert rel(amu, g["amjdp_amu"]) < 1e-13
+
+
+def test_push_and_reorder(g):
+    nx, ny, dex = int(g["nx"]), int(g["ny"]), float(g["dex"])
+    part = g["extpsi_out"].copy()
+    _, bxy, psit = (g[f"amjdp_f_{i}"] for i in range(3))
+    err = qpo.push(part, g["push_fxy"], bxy, psit, -1.0, float(g["push_dt"]), dex)
+    assert err == 0
+    kp = qpo.order_by_tile(nx, ny, part)
+    want_k, want = g["push_kpic"], g["push_part"]
+    assert np.array_equal(kp, want_k)
+    for k in range(len(kp)):
+        s0, s1 = kp[:k].sum(), kp[:k + 1].sum()
+        a = part[s0:s1][np.lexsort(part[s0:s1, ::-1].T)]
+        b = want[s0:s1][np.lexsort(want[s0:s1, ::-1].T)]
+        assert np.array_equal(a, b), k
+
+
+def test_ranorm_and_beam_loader(g):
+    assert np.array_equal(qpo.ranorm_stream(4096), g["ranorm"])
+    grid = qpo.Grid(5, 5, 5)
+    spec = qpo.BeamSpec(np_=(8, 8, 16), peak=4.0, sigma=(1.5, 1.5, 1.5), center=(0.0, 0.0, 5.0))
+    part, qm, ierr = qpo.beam_init(grid, spec)
+    assert ierr == 0 and qm == float(g["beam_qm"])
+    assert np.array_equal(part, g["beam_part"])
+
+
+def test_beam_deposit_and_push(g):
+    grid = qpo.Grid(5, 5, 5)
+    part = g["beam_part"]
+    q3 = np.zeros((1, grid.nz + 1, grid.ny + 1, grid.nx + 1))
+    qpo.beam_deposit(part, q3)
+    assert rel(q3[0], g["beam_q3"]) < 1e-14
+    e3 = g["beam_e3"].astype(np.float64)
+    b3 = g["beam_b3"].astype(np.float64)
+    mine = qpo.beam_push(part.copy(), e3, b3, -1.0, float(g["beam_dt"]), grid.dx, grid.dz)
+    assert 0 < mine.shape[0] < part.shape[0]
+    assert np.array_equal(mine, g["beam_push"])
