@@ -28,6 +28,24 @@ for _p in (REPO, PKG):
 METRIC = ("ξ-slices/s at 1024², 16 ppc; CNN-seeded FP32 vs plain FP64 at same B "
           "residual")
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(REPO, "profiles", "slice_loop_fp32_1024sq_16ppc.md")
+KERNEL_SYMBOL = {"amjdp": "k_amjdp", "push": "k_push", "extpsi": "k_extpsi",
+                 "deposit_q": "k_deposit_q", "sort": "k_scatter"}
+
+
+def profiled_traffic(kind, precision_name):
+    """HBM bytes per launch of a kernel class as rocprofv3 --pmc measured it
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size), from the
+    committed profile summary; None when that summary has no such entry."""
+    if not os.path.exists(PROFILE):
+        return None
+    want = f"qp::{KERNEL_SYMBOL[kind]}<{precision_name}"
+    with open(PROFILE) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1].startswith(want):
+                return float(parts[2])
+    return None
 
 
 def log(*a):
@@ -124,8 +142,9 @@ def run_gpu(args, precision, beam_part=None):
     if cnt:
         avg_s = t / cnt / 1e3
         ach = (b / cnt) / avg_s / 1e9
+        traffic = profiled_traffic(dom, "float" if precision == qps.FP32 else "double")
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_ms": round(t / cnt, 4), "algorithmic_bytes_per_launch": b / cnt}
     total_ms = sum(v[0] for v in prof.values())
     nprof = max(nprof, 1)

@@ -25,7 +25,8 @@ struct qp_species {
     double qbm, dt;
     int64_t np;
     qp::DevBuf a[8], b[8];     // x, y, px, py, pz, p6, p7, q (current, alternate)
-    qp::DevBuf key, slot;      // per particle: destination cell key and rank
+    qp::DevBuf key, slot;      // per particle: destination bin and rank (push), key and index (load)
+    qp::DevBuf bins;           // per tile: counts, then starts, of the 9 x 256 destination bins
     qp::DevBuf count, off;     // per cell (ncells + 1)
     qp::DevBuf scan_tmp, err;
     size_t scan_bytes;
@@ -428,126 +429,199 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
 }
 
 // PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
-// periodic wrap of the reorder (PPPORDER2LB :1936-1977); one thread per
-// particle.  Emits every particle's new cell key and its rank in that cell;
-// lanes of a wave bound for the same cell take one atomic between them.  A
-// particle whose new tile is not its old tile plus the leaving direction (the
-// condition PPPCHECK2L reports) sets *err.
+// periodic wrap of the reorder (PPPORDER2LB :1936-1977).  One workgroup per
+// source tile walks the tile's particles (a contiguous run of the sorted
+// arrays) 256 at a time, one thread per particle.  A particle leaves for the
+// neighbour tile in its direction of motion (dir in {-1,0,1}^2, bin
+// nb = 3 (diry+1) + dirx + 1) and lands in cell cl of it; its rank among the
+// tile's particles bound for the same (nb, cl) follows their order in the
+// tile: lanes of a wave rank by lane order, waves add their counts to the
+// tile's LDS histogram in turn.  So the new order is a function of the old
+// one, with no atomics: the sort, and with it every later sum, is the same
+// run to run.  The histogram goes to cnt[tile][nb][cl].  A particle whose new
+// tile is not its old tile plus its direction (the condition PPPCHECK2L
+// reports) sets *err and is kept in its own tile.
+constexpr int NBINS = 9 * TCELLS;
+
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_push(int64_t np, TileGeom g, Soa<R> s, const R *fxy, const R *bxy, const R *psit, size_t cs,
-       R qbm, R dt, R dex, int *newkey, int *slot, int *count, int *err)
+k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
+       const R *psit, size_t cs, R qbm, R dt, R dex, int *bin, int *rank, int *cnt, int *err)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool valid = i < np;
-    int k = -1;
-    if (valid) {
-        R x = s.x[i], y = s.y[i];
-        const int cx = min(max((int)x, 0), g.nx - 1), cy = min(max((int)y, 0), g.ny - 1);
-        const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
-        const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
-        const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
-        const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
-        const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
-        const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
-        const R idex = R(1) / dex, qtmh = R(0.5) * qbm * dt;
-        const R anx = (R)g.nx, any = (R)g.ny;
-        const int tx0 = cx >> 4, ty0 = cy >> 4;
-        const R edgelx = (R)(tx0 * TILE), edgely = (R)(ty0 * TILE);
-        const R edgerx = edgelx + (R)min(TILE, g.nx - tx0 * TILE);
-        const R edgery = edgely + (R)min(TILE, g.ny - ty0 * TILE);
-        R p6 = s.p6[i], p7 = s.p7[i];
-        R qtmh1 = qtmh / p7;
-        R qtmh2 = qtmh1 * p6;
-        R dxp = x - (R)cx, dyp = y - (R)cy;
-        R amx = R(1) - dxp, amy = R(1) - dyp;
-        R dx = gather(ex, amx, amy, dxp, dyp);
-        R dy = gather(ey, amx, amy, dxp, dyp);
-        R dz = gather(ez, amx, amy, dxp, dyp);
-        dz = dz * idex;
-        R ox = gather(bx, amx, amy, dxp, dyp);
-        R oy = gather(by, amx, amy, dxp, dyp);
-        R oz = gather(bz, amx, amy, dxp, dyp);
-        ox = ox * dex;
-        oy = oy * dex;
-        dx = qtmh2 * dx;
-        dy = qtmh2 * dy;
-        dz = qtmh2 * dz;
-        R acx = s.px[i] + dx, acy = s.py[i] + dy, acz = s.pz[i] + dz;
-        R omxt = qtmh1 * ox, omyt = qtmh1 * oy, omzt = qtmh1 * oz;
-        R omt = omxt * omxt + omyt * omyt + omzt * omzt;
-        R anorm = R(2) / (R(1) + omt);
-        omt = R(0.5) * (R(1) - omt);
-        R rot4 = omxt * omyt, rot7 = omxt * omzt, rot8 = omyt * omzt;
-        R rot1 = omt + omxt * omxt;
-        R rot5 = omt + omyt * omyt;
-        R rot9 = omt + omzt * omzt;
-        R rot2 = omzt + rot4;
-        rot4 = -omzt + rot4;
-        R rot3 = -omyt + rot7;
-        rot7 = omyt + rot7;
-        R rot6 = omxt + rot8;
-        rot8 = -omxt + rot8;
-        dx = (rot1 * acx + rot2 * acy + rot3 * acz) * anorm + dx;
-        dy = (rot4 * acx + rot5 * acy + rot6 * acz) * anorm + dy;
-        dz = (rot7 * acx + rot8 * acy + rot9 * acz) * anorm + dz;
-        s.px[i] = dx;
-        s.py[i] = dy;
-        s.pz[i] = dz;
-        R dtc1 = dt / (sqrt(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
-        dx = x + dx * dtc1;
-        dy = y + dy * dtc1;
-        int dirx = 0, diry = 0;
-        if (dx >= edgerx) dirx = 1;
-        else if (dx < edgelx) {
-            dirx = -1;
-            if (dx < R(0) && dx + anx >= anx) { dx = R(0); dirx = 0; }
+    __shared__ int hist[NBINS];
+    const int S = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k = tid; k < NBINS; k += TCELLS) hist[k] = 0;
+    const int tx0 = S % g.mx1, ty0 = S / g.mx1;
+    const int p0 = off[S * TCELLS], p1 = off[(S + 1) * TCELLS];
+    const R idex = R(1) / dex, qtmh = R(0.5) * qbm * dt;
+    const R anx = (R)g.nx, any = (R)g.ny;
+    const R edgelx = (R)(tx0 * TILE), edgely = (R)(ty0 * TILE);
+    const R edgerx = edgelx + (R)min(TILE, g.nx - tx0 * TILE);
+    const R edgery = edgely + (R)min(TILE, g.ny - ty0 * TILE);
+    for (int c0 = p0; c0 < p1; c0 += TCELLS) {
+        const int i = c0 + tid;
+        const bool valid = i < p1;
+        int b = 0;
+        if (valid) {
+            R x = s.x[i], y = s.y[i];
+            const int cx = min(max((int)x, 0), g.nx - 1), cy = min(max((int)y, 0), g.ny - 1);
+            const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
+            const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
+            const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
+            const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
+            const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
+            const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
+            R p6 = s.p6[i], p7 = s.p7[i];
+            R qtmh1 = qtmh / p7;
+            R qtmh2 = qtmh1 * p6;
+            R dxp = x - (R)cx, dyp = y - (R)cy;
+            R amx = R(1) - dxp, amy = R(1) - dyp;
+            R dx = gather(ex, amx, amy, dxp, dyp);
+            R dy = gather(ey, amx, amy, dxp, dyp);
+            R dz = gather(ez, amx, amy, dxp, dyp);
+            dz = dz * idex;
+            R ox = gather(bx, amx, amy, dxp, dyp);
+            R oy = gather(by, amx, amy, dxp, dyp);
+            R oz = gather(bz, amx, amy, dxp, dyp);
+            ox = ox * dex;
+            oy = oy * dex;
+            dx = qtmh2 * dx;
+            dy = qtmh2 * dy;
+            dz = qtmh2 * dz;
+            R acx = s.px[i] + dx, acy = s.py[i] + dy, acz = s.pz[i] + dz;
+            R omxt = qtmh1 * ox, omyt = qtmh1 * oy, omzt = qtmh1 * oz;
+            R omt = omxt * omxt + omyt * omyt + omzt * omzt;
+            R anorm = R(2) / (R(1) + omt);
+            omt = R(0.5) * (R(1) - omt);
+            R rot4 = omxt * omyt, rot7 = omxt * omzt, rot8 = omyt * omzt;
+            R rot1 = omt + omxt * omxt;
+            R rot5 = omt + omyt * omyt;
+            R rot9 = omt + omzt * omzt;
+            R rot2 = omzt + rot4;
+            rot4 = -omzt + rot4;
+            R rot3 = -omyt + rot7;
+            rot7 = omyt + rot7;
+            R rot6 = omxt + rot8;
+            rot8 = -omxt + rot8;
+            dx = (rot1 * acx + rot2 * acy + rot3 * acz) * anorm + dx;
+            dy = (rot4 * acx + rot5 * acy + rot6 * acz) * anorm + dy;
+            dz = (rot7 * acx + rot8 * acy + rot9 * acz) * anorm + dz;
+            s.px[i] = dx;
+            s.py[i] = dy;
+            s.pz[i] = dz;
+            R dtc1 = dt / (sqrt(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
+            dx = x + dx * dtc1;
+            dy = y + dy * dtc1;
+            int dirx = 0, diry = 0;
+            if (dx >= edgerx) dirx = 1;
+            else if (dx < edgelx) {
+                dirx = -1;
+                if (dx < R(0) && dx + anx >= anx) { dx = R(0); dirx = 0; }
+            }
+            if (dy >= edgery) diry = 1;
+            else if (dy < edgely) {
+                diry = -1;
+                if (dy < R(0) && dy + any >= any) { dy = R(0); diry = 0; }
+            }
+            if (dx < R(0)) dx = dx + anx;
+            if (dx >= anx) dx = dx - anx;
+            if (dy < R(0)) dy = dy + any;
+            if (dy >= any) dy = dy - any;
+            s.x[i] = dx;
+            s.y[i] = dy;
+            int ncx = (int)dx, ncy = (int)dy;
+            // a non-finite or out-of-range position must not become an index
+            if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
+                atomicMax(err, 1 << 30);
+                ncx = min(max(ncx, 0), g.nx - 1);
+                ncy = min(max(ncy, 0), g.ny - 1);
+            }
+            const int txd = (tx0 + dirx + g.mx1) % g.mx1, tyd = (ty0 + diry + g.my1) % g.my1;
+            if ((ncx >> 4) != txd || (ncy >> 4) != tyd) {
+                atomicMax(err, 1 + txd + g.mx1 * tyd);
+                dirx = diry = 0;
+            }
+            b = (3 * (diry + 1) + dirx + 1) * TCELLS + ((ncy & 15) << 4) + (ncx & 15);
         }
-        if (dy >= edgery) diry = 1;
-        else if (dy < edgely) {
-            diry = -1;
-            if (dy < R(0) && dy + any >= any) { dy = R(0); diry = 0; }
+        // rank among this chunk's earlier particles of the same bin, in lane order
+        unsigned long long todo = __ballot(valid);
+        int rk = 0, nlead = 0;
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lb = __shfl(b, leader);
+            const unsigned long long m = __ballot(valid && b == lb);
+            if (valid && b == lb) rk = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == leader) nlead = __popcll(m);
+            todo &= ~m;
         }
-        if (dx < R(0)) dx = dx + anx;
-        if (dx >= anx) dx = dx - anx;
-        if (dy < R(0)) dy = dy + any;
-        if (dy >= any) dy = dy - any;
-        s.x[i] = dx;
-        s.y[i] = dy;
-        int ncx = (int)dx, ncy = (int)dy;
-        // a non-finite or out-of-range position must not become an index
-        if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
-            atomicMax(err, 1 << 30);
-            ncx = min(max(ncx, 0), g.nx - 1);
-            ncy = min(max(ncy, 0), g.ny - 1);
+        // waves add to the tile histogram in order
+        for (int w = 0; w < TCELLS / 64; ++w) {
+            __syncthreads();
+            if (wave == w) {
+                if (valid) rk += hist[b];
+                if (nlead) hist[b] += nlead;
+            }
         }
-        int txd = (tx0 + dirx + g.mx1) % g.mx1, tyd = (ty0 + diry + g.my1) % g.my1;
-        if ((ncx >> 4) != txd || (ncy >> 4) != tyd) atomicMax(err, 1 + txd + g.mx1 * tyd);
-        k = cell_key(ncx, ncy, g.mx1);
-        newkey[i] = k;
+        if (valid) {
+            bin[i] = S * NBINS + b;
+            rank[i] = rk;
+        }
+        __syncthreads();
     }
-    // rank in the destination cell: one atomic per distinct key in the wave
-    const int lane = threadIdx.x & 63;
-    unsigned long long todo = __ballot(valid);
-    int myslot = 0;
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int lk = __shfl(k, leader);
-        const unsigned long long m = __ballot(valid && k == lk);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[lk], __popcll(m));
-        base = __shfl(base, leader);
-        if (valid && k == lk) myslot = base + __popcll(m & ((1ull << lane) - 1ull));
-        todo &= ~m;
-    }
-    if (valid) slot[i] = myslot;
+    __syncthreads();
+    for (int k = tid; k < NBINS; k += TCELLS) cnt[(size_t)S * NBINS + k] = hist[k];
 }
 
+// Per destination cell d (in tile T, local cell cl): its particles come from
+// bins (T - dir(nb), nb, cl), nb = 0..8.  Taken in nb order, the bin counts
+// are turned into each bin's start within the cell, and the cell count is
+// written for the scan; count[ncells] = 0.
+__global__ void k_bin_starts(TileGeom g, int *cnt, int *count)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncells = g.mx1 * g.my1 * TCELLS;
+    if (d == ncells) count[d] = 0;
+    if (d >= ncells) return;
+    const int T = d >> 8, cl = d & 255;
+    const int tx = T % g.mx1, ty = T / g.mx1;
+    int run = 0;
+    for (int nb = 0; nb < 9; ++nb) {
+        const int dirx = nb % 3 - 1, diry = nb / 3 - 1;
+        const int sx = (tx - dirx + g.mx1) % g.mx1, sy = (ty - diry + g.my1) % g.my1;
+        const size_t k = (size_t)(sy * g.mx1 + sx) * NBINS + nb * TCELLS + cl;
+        const int c = cnt[k];
+        cnt[k] = run;
+        run += c;
+    }
+    count[d] = run;
+}
 
-// cell keys of unsorted particles (initial load and uploads)
+// destination of particle i: off[cell] + start of its bin in the cell + rank
+template <typename R>
+__global__ void k_scatter(int64_t np, Soa<R> a, Soa<R> b, const int *bin, const int *rank,
+                          const int *cnt, const int *off, int mx1, int my1)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int bi = bin[i];
+    const int S = bi / NBINS, bb = bi - S * NBINS, nb = bb >> 8, cl = bb & 255;
+    const int tx = (S % mx1 + nb % 3 - 1 + mx1) % mx1, ty = (S / mx1 + nb / 3 - 1 + my1) % my1;
+    const int d = off[(ty * mx1 + tx) * TCELLS + cl] + cnt[bi] + rank[i];
+    b.x[d] = a.x[i];
+    b.y[d] = a.y[i];
+    b.px[d] = a.px[i];
+    b.py[d] = a.py[i];
+    b.pz[d] = a.pz[i];
+    b.p6[d] = a.p6[i];
+    b.p7[d] = a.p7[i];
+    b.q[d] = a.q[i];
+}
+
+// cell keys of unsorted particles (initial load and uploads), with their
+// index for a stable sort and a per-cell count
 template <typename R>
 __global__ void k_keys(int64_t np, const R *x, const R *y, int nx, int ny, int mx1, int *key,
-                       int *slot, int *count, int *err)
+                       int *idx, int *count, int *err)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -560,24 +634,25 @@ __global__ void k_keys(int64_t np, const R *x, const R *y, int nx, int ny, int m
     }
     int k = cell_key(cx, cy, mx1);
     key[i] = k;
-    slot[i] = atomicAdd(&count[k], 1);
+    idx[i] = (int)i;
+    atomicAdd(&count[k], 1);
 }
 
+// b[p] = a[src[p]]
 template <typename R>
-__global__ void k_scatter(int64_t np, Soa<R> a, Soa<R> b, const int *key, const int *slot,
-                          const int *off)
+__global__ void k_gather(int64_t np, Soa<R> a, Soa<R> b, const int *src)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    int d = off[key[i]] + slot[i];
-    b.x[d] = a.x[i];
-    b.y[d] = a.y[i];
-    b.px[d] = a.px[i];
-    b.py[d] = a.py[i];
-    b.pz[d] = a.pz[i];
-    b.p6[d] = a.p6[i];
-    b.p7[d] = a.p7[i];
-    b.q[d] = a.q[i];
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int i = src[p];
+    b.x[p] = a.x[i];
+    b.y[p] = a.y[i];
+    b.px[p] = a.px[i];
+    b.py[p] = a.py[i];
+    b.pz[p] = a.pz[i];
+    b.p6[p] = a.p6[i];
+    b.p7[p] = a.p7[i];
+    b.q[p] = a.q[i];
 }
 
 __global__ void k_zero_int(int *p, int n)
@@ -589,22 +664,13 @@ static TileGeom geom(qp_ctx *c) { return TileGeom{c->nx, c->ny, c->mx1, c->my1, 
 
 static dim3 pgrid(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-// counts -> offsets -> scatter into the alternate arrays, then swap
-template <typename R>
-static int rebin_tail(qp_species *sp)
+// cell offsets from the per-cell counts
+static int scan_counts(qp_species *sp)
 {
     qp_ctx *c = sp->ctx;
     size_t tb = sp->scan_bytes;
     QP_HIP(hipcub::DeviceScan::ExclusiveSum(sp->scan_tmp.p, tb, sp->count.as<int>(),
                                             sp->off.as<int>(), c->ncells + 1, c->stream));
-    {
-    ProfScope _ps(c, QP_PROF_SORT, 16.0 * sizeof(R) * (double)sp->np);
-    hipLaunchKernelGGL(k_scatter<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
-                       soa<R>(sp->a), soa<R>(sp->b), sp->key.as<int>(), sp->slot.as<int>(),
-                       sp->off.as<int>());
-    }
-    QP_LAUNCH_CHECK();
-    for (int k = 0; k < 8; ++k) std::swap(sp->a[k], sp->b[k]);
     return QP_OK;
 }
 
@@ -632,22 +698,44 @@ static int check_err(qp_species *sp, const char *what)
     return QP_OK;
 }
 
+// sort an arbitrary particle order by cell: keys, a stable radix sort of
+// (key, index), offsets, then a gather into the alternate arrays
 template <typename R>
 static int rebin_unsorted(qp_species *sp)
 {
     qp_ctx *c = sp->ctx;
     int rc = zero_counts(sp);
     if (rc) return rc;
+    const int64_t np = sp->np;
+    DevBuf keys2, idx2, tmp;
+    rc = keys2.alloc((size_t)np * 4 + 4);
+    if (!rc) rc = idx2.alloc((size_t)np * 4 + 4);
+    if (rc) return rc;
+    size_t tb = 0;
+    int bits = 1;
+    while ((1 << bits) < c->ncells) ++bits;
+    QP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, sp->key.as<int>(), keys2.as<int>(),
+                                              sp->slot.as<int>(), idx2.as<int>(), (int)np, 0,
+                                              bits, c->stream));
+    rc = tmp.alloc(tb + 256);
+    if (rc) return rc;
     {
-    ProfScope _ps(c, QP_PROF_SORT, 2.0 * sizeof(R) * (double)sp->np);
-    hipLaunchKernelGGL(k_keys<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
-                       sp->a[0].as<R>(), sp->a[1].as<R>(), c->nx, c->ny, c->mx1,
-                       sp->key.as<int>(), sp->slot.as<int>(), sp->count.as<int>(),
-                       sp->err.as<int>());
+    ProfScope _ps(c, QP_PROF_SORT, 2.0 * sizeof(R) * (double)np);
+    hipLaunchKernelGGL(k_keys<R>, pgrid(np), dim3(256), 0, c->stream, np, sp->a[0].as<R>(),
+                       sp->a[1].as<R>(), c->nx, c->ny, c->mx1, sp->key.as<int>(),
+                       sp->slot.as<int>(), sp->count.as<int>(), sp->err.as<int>());
     }
     QP_LAUNCH_CHECK();
-    rc = rebin_tail<R>(sp);
+    QP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, sp->key.as<int>(), keys2.as<int>(),
+                                              sp->slot.as<int>(), idx2.as<int>(), (int)np, 0,
+                                              bits, c->stream));
+    rc = scan_counts(sp);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_gather<R>, pgrid(np), dim3(256), 0, c->stream, np, soa<R>(sp->a),
+                       soa<R>(sp->b), idx2.as<const int>());
+    QP_LAUNCH_CHECK();
+    for (int k = 0; k < 8; ++k) std::swap(sp->a[k], sp->b[k]);
+    QP_HIP(hipStreamSynchronize(c->stream));   // the temporaries are freed on return
     return check_err(sp, "particle load");
 }
 
@@ -683,7 +771,6 @@ template <> struct DepTab<double> {
     static constexpr DepShape j[] = {{8, 1024, 1}, {4, 1024, 1}, {4, 512, 1}, {8, 512, 1},
                                      {2, 512, 1}, {16, 512, 1}};
 };
-constexpr int NQVAR = 4, NJVAR = 6;
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
 template <typename K>
@@ -792,6 +879,7 @@ static int alloc_particles(qp_species *sp, int64_t np)
     int rc = sp->key.alloc((size_t)np * 4);
     if (!rc) rc = sp->slot.alloc((size_t)np * 4);
     if (!rc) rc = sp->count.alloc((size_t)(c->ncells + 1) * 4);
+    if (!rc) rc = sp->bins.alloc((size_t)c->ntiles * NBINS * 4);
     if (!rc) rc = sp->off.alloc((size_t)(c->ncells + 1) * 4);
     if (!rc) rc = sp->err.alloc(sizeof(int));
     if (rc) return rc;
@@ -876,18 +964,31 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                   const qp_field2d *psit, double dex, bool check)
 {
     qp_ctx *c = sp->ctx;
-    int rc = zero_counts(sp);
-    if (rc) return rc;
-    if (sp->np > 0) {
-        ProfScope _ps(c, QP_PROF_PUSH, (12.0 * sp->np + 6.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
-        hipLaunchKernelGGL(k_push<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np, geom(c),
-                           soa<R>(sp->a), ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(),
-                           c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex, sp->key.as<int>(),
-                           sp->slot.as<int>(), sp->count.as<int>(), sp->err.as<int>());
+    const TileGeom g = geom(c);
+    {
+    ProfScope _ps(c, QP_PROF_PUSH, (12.0 * sp->np + 6.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
+    hipLaunchKernelGGL(k_push<R>, dim3(c->ntiles), dim3(TCELLS), 0, c->stream, g, soa<R>(sp->a),
+                       sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(),
+                       c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex, sp->key.as<int>(),
+                       sp->slot.as<int>(), sp->bins.as<int>(), sp->err.as<int>());
     }
     QP_LAUNCH_CHECK();
-    rc = rebin_tail<R>(sp);
+    {
+    ProfScope _ps(c, QP_PROF_SORT, 8.0 * (double)c->ntiles * NBINS);
+    hipLaunchKernelGGL(k_bin_starts, dim3(c->ncells / 256 + 1), dim3(256), 0, c->stream, g,
+                       sp->bins.as<int>(), sp->count.as<int>());
+    }
+    QP_LAUNCH_CHECK();
+    int rc = scan_counts(sp);
     if (rc) return rc;
+    {
+    ProfScope _ps(c, QP_PROF_SORT, 18.0 * sizeof(R) * (double)sp->np);
+    hipLaunchKernelGGL(k_scatter<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
+                       soa<R>(sp->a), soa<R>(sp->b), sp->key.as<int>(), sp->slot.as<int>(),
+                       sp->bins.as<int>(), sp->off.as<int>(), c->mx1, c->my1);
+    }
+    QP_LAUNCH_CHECK();
+    for (int k = 0; k < 8; ++k) std::swap(sp->a[k], sp->b[k]);
     return check ? check_err(sp, "pmove") : QP_OK;
 }
 
@@ -936,6 +1037,7 @@ int qp_species_destroy(qp_species *sp)
     if (sp->ctx) (void)hipStreamSynchronize(sp->ctx->stream);
     for (int k = 0; k < 8; ++k) { sp->a[k].release(); sp->b[k].release(); }
     sp->key.release(); sp->slot.release(); sp->count.release(); sp->off.release();
+    sp->bins.release();
     sp->scan_tmp.release(); sp->err.release();
     if (sp->h_err) (void)hipHostFree(sp->h_err);
     if (sp->err_ev) (void)hipEventDestroy(sp->err_ev);
