@@ -68,6 +68,8 @@ def run_gpu(args, precision, beam_part=None):
     if beam_part is not None:
         bm.upload(beam_part)
     sim = qps.Sim(ctx, [sp], [bm], iter_=2)
+    # the same beam for the FP64 leg
+    part = bm.download() if beam_part is None and args.fp64 else None
     log(f"[bench] {'FP32' if precision == qps.FP32 else 'FP64'} setup {time.time() - t0:.1f}s "
         f"(plasma {sp.np}, beam {bm.np})")
     sim.begin_step()
@@ -161,7 +163,23 @@ def run_gpu(args, precision, beam_part=None):
         "np_beam": bm.np,
         "sweep": sweep,
     }
-    part = bm.download() if beam_part is None and args.fp64 else None
+    # finish the 3-D step (untimed): the remaining slices and the beam push;
+    # sample the wake at a few slices for the FP32-vs-FP64 comparison
+    j0 = j_first + args.steps + nprof + (40 if args.sweep else 0)
+    for j in range(j0, ctx.nz + 1):
+        sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+    g0 = bm.moments()[0]
+    lpos = sorted({min(ctx.nz + 1, j_first + 1 + k * (ctx.nz - j_first) // 4) for k in range(5)})
+    e3, b3 = sim.bexyz, sim.bbxyz
+    result["wake"] = {
+        "Ez": np.stack([e3.plane(3, lp) for lp in lpos]),
+        "Ex-By": np.stack([e3.plane(1, lp) - b3.plane(2, lp) for lp in lpos]),
+        "Bz": np.stack([b3.plane(3, lp) for lp in lpos]),
+        "psi": sim.field(qps.F_PSI).download(0),
+    }
+    sim.end_step()
+    g1 = bm.moments()[0]
+    result["beam_energy"] = (g0, g1)
     ctx.close()
     return result, part
 
@@ -232,6 +250,8 @@ def main():
     if args.calib_copy:
         calib_copy(args)
         return
+    import numpy as np
+
     import qpslice as qps
     r32, beam = run_gpu(args, qps.FP32)
     log(f"[bench] FP32: {r32['slices_per_s']:.1f} slices/s, passes {r32['passes_mean']:.2f}")
@@ -239,6 +259,16 @@ def main():
     if args.fp64:
         r64, _ = run_gpu(args, qps.FP64, beam_part=beam)
         log(f"[bench] FP64: {r64['slices_per_s']:.1f} slices/s, passes {r64['passes_mean']:.2f}")
+    errs = None
+    if r64 is not None:
+        # relative error of the precision form against the FP64 path on the
+        # same beam: max |a32 - a64| / max |a64| over the sampled wake slices
+        errs = {k: float(np.abs(r32["wake"][k] - r64["wake"][k]).max()
+                         / max(np.abs(r64["wake"][k]).max(), 1e-300)) for k in r32["wake"]}
+        (a0, a1), (b0, b1) = r32["beam_energy"], r64["beam_energy"]
+        errs["beam energy"] = abs(a1 - b1) / abs(b1)
+        errs["beam energy change"] = abs((a1 - a0) - (b1 - b0)) / max(abs(b1 - b0), 1e-300)
+        log(f"[bench] FP32 vs FP64 relative error: {errs}")
     cpu = cpu_baseline(args) if args.cpu else None
     out = {
         "metric": METRIC,
@@ -275,6 +305,7 @@ def main():
                              "ms_per_step": round(r64["ms_per_slice"], 4),
                              "passes_per_slice_mean": r64["passes_mean"],
                              "fp32_over_fp64": round(r32["slices_per_s"] / r64["slices_per_s"], 3),
+                             "fp32_rel_err": {k: float(f"{v:.3g}") for k, v in errs.items()},
                              "roofline": r64["roofline"]}
     print(json.dumps(out), flush=True)
 
