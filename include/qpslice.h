@@ -101,8 +101,13 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * differ only in how a cell's particles are split over lanes, so results
  * agree to rounding.  QP_TUNE_SPECULATE = 1 lets qp_slice_run enqueue
  * corrector pass l+1 before reading the residual of pass l (default 0: wait
- * for each residual).  value -1 restores a knob's default. */
-enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_COUNT };
+ * for each residual).  QP_TUNE_SEED = 1 seeds the corrector of a slice run
+ * with a residual tolerance by linear extrapolation 2 B(xi-1) - B(xi-2)
+ * instead of the reference's B(xi-1) (default 0); it changes only how many
+ * passes reach the tolerance, and fixed-pass runs (tol <= 0) ignore it.
+ * value -1 restores a knob's default. */
+enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_SEED = 3,
+       QP_TUNE_COUNT };
 int qp_ctx_tune(qp_ctx *ctx, int knob, int value);
 /* current value and the number of values of a knob */
 int qp_ctx_tune_get(const qp_ctx *ctx, int knob, int *value, int *nvalues);

@@ -229,6 +229,7 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
 // when tol > 0 and it is below tol, the converged flag is set.
 int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol);
 int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
+int fused_seed(qp_field2d *epw, qp_field2d *hist, int extrap);
 int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
 int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
                       qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex);

@@ -17,8 +17,9 @@
 // (M/8 threads per transform, 8 points each) and one closing radix-2 or
 // radix-4 pass when log2(M) is not a multiple of 3.  The first pass reads the
 // extension straight from memory, the last writes the n+1 outputs straight
-// to memory, and the passes between exchange through LDS; all twiddles are
-// fetched before the first butterfly.  A batch holds up to 8 components of
+// to memory, and the passes between exchange through LDS; the twiddle table
+// entries are fetched before the first butterfly (three per radix-8 pass,
+// the other powers formed by complex products).  A batch holds up to 8 components of
 // any fields (grid.y); an LDS-tiled transpose sits between the x and the y
 // pass.
 #include "qp_common.h"
@@ -107,24 +108,29 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     const bool live = ra < nrows, hb = ra + 1 < nrows;
     cplx<R> *buf = reinterpret_cast<cplx<R> *>(smem_raw) + f * S::LDSM;
 
-    // twiddles of every pass after the first: pass p (Ns = 8^p) needs
-    // W[r e], e = (t mod Ns) M / (8 Ns); the closing pass W[r j]
-    constexpr int NTW8 = S::NR8 > 1 ? 7 * (S::NR8 - 1) : 1;
+    // twiddles of every pass after the first, all fetched up front: pass p
+    // (Ns = 8^p) needs W^(r e), e = (t mod Ns) M / (8 Ns), r = 1..7: W^e,
+    // W^2e and W^4e come from the table, the other four are their products;
+    // the closing radix-4 pass needs W^j, W^2j (table) and W^3j, the
+    // closing radix-2 pass W^j
+    constexpr int NTW8 = S::NR8 > 1 ? 3 * (S::NR8 - 1) : 1;
     cplx<R> tw8[NTW8];
 #pragma unroll
     for (int p = 1; p < S::NR8; ++p) {
         const int Ns = 1 << (3 * p);
         const int e = (t & (Ns - 1)) * (M / (8 * Ns));
-#pragma unroll
-        for (int r = 1; r < 8; ++r) tw8[7 * (p - 1) + r - 1] = W[r * e];
+        tw8[3 * (p - 1)] = W[e];
+        tw8[3 * (p - 1) + 1] = W[2 * e];
+        tw8[3 * (p - 1) + 2] = W[4 * e];
     }
-    constexpr int NTWR = S::REM == 2 ? 6 : (S::REM == 1 ? 4 : 1);
+    constexpr int NTWR = S::REM == 2 ? 4 : (S::REM == 1 ? 4 : 1);
     cplx<R> twr[NTWR];
     if (S::REM == 2) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int r = 1; r < 4; ++r) twr[3 * s + r - 1] = W[r * (t + s * TF)];
+        for (int s = 0; s < 2; ++s) {
+            twr[2 * s] = W[t + s * TF];
+            twr[2 * s + 1] = W[2 * (t + s * TF)];
+        }
     } else if (S::REM == 1) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) twr[s] = W[t + s * TF];
@@ -189,10 +195,16 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     for (int p = 1; p < S::NR8; ++p) {
         const int Ns = 1 << (3 * p);
         __syncthreads();
+        {
+            const cplx<R> w1 = tw8[3 * (p - 1)], w2 = tw8[3 * (p - 1) + 1],
+                          w4 = tw8[3 * (p - 1) + 2];
+            const cplx<R> w3 = cmul(w1, w2);
+            const cplx<R> w[8] = {w1, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4)};
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            v[r] = buf[pad8(t + r * TF)];
-            if (r) v[r] = cmul(v[r], tw8[7 * (p - 1) + r - 1]);
+            for (int r = 0; r < 8; ++r) {
+                v[r] = buf[pad8(t + r * TF)];
+                if (r) v[r] = cmul(v[r], w[r]);
+            }
         }
         dft8(v);
         if (p == S::NR8 - 1 && S::REM == 0) {
@@ -212,10 +224,11 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int j = t + s * TF;
+            const cplx<R> w1 = twr[2 * s], w2 = twr[2 * s + 1];
             cplx<R> a0 = buf[pad8(j)];
-            cplx<R> a1 = cmul(buf[pad8(j + M / 4)], twr[3 * s]);
-            cplx<R> a2 = cmul(buf[pad8(j + M / 2)], twr[3 * s + 1]);
-            cplx<R> a3 = cmul(buf[pad8(j + 3 * M / 4)], twr[3 * s + 2]);
+            cplx<R> a1 = cmul(buf[pad8(j + M / 4)], w1);
+            cplx<R> a2 = cmul(buf[pad8(j + M / 2)], w2);
+            cplx<R> a3 = cmul(buf[pad8(j + 3 * M / 4)], cmul(w1, w2));
             dft4(a0, a1, a2, a3);
             emit(j, a0);
             emit(j + M / 4, a1);

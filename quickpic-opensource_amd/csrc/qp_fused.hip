@@ -129,6 +129,20 @@ k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *_
     }
 }
 
+// Corrector seed by linear extrapolation in xi: epw <- 2 epw - hist with
+// hist <- epw (extrap = 1), or only hist <- epw (extrap = 0).  Only the
+// iteration count to a given residual depends on the seed.
+template <typename R>
+__global__ void k_seed(size_t n, R *epw, R *hist, int extrap)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const R e1 = epw[e];
+        if (extrap) epw[e] = R(2) * e1 - hist[e];
+        hist[e] = e1;
+    }
+}
+
 // after the loop (simulation_class.f03:725-726): dcu *= dxi; cu(1:2) -= dcu(1:2)
 template <typename R>
 __global__ void k_predict_cu(size_t n, size_t cs, R *cu, R *dcu, R dxi)
@@ -238,6 +252,26 @@ int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol)
 {
     return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, l, tol)
                                       : pass_t<float>(f, dex, qp, l, tol);
+}
+
+template <typename R>
+static int seed_t(qp_field2d *epw, qp_field2d *hist, int extrap)
+{
+    qp_ctx *c = epw->ctx;
+    const size_t n = (size_t)epw->dim * c->kplane;
+    {
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, (extrap ? 4.0 : 2.0) * n * sizeof(R));
+    hipLaunchKernelGGL(k_seed<R>, flat_grid(n), dim3(256), 0, c->stream, n, epw->ks.as<R>(),
+                       hist->ks.as<R>(), extrap);
+    }
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int fused_seed(qp_field2d *epw, qp_field2d *hist, int extrap)
+{
+    return epw->ctx->prec == QP_FP64 ? seed_t<double>(epw, hist, extrap)
+                                     : seed_t<float>(epw, hist, extrap);
 }
 
 template <typename R>

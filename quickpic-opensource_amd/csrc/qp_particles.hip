@@ -296,6 +296,82 @@ k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R d
     s.p6[i] = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
 }
 
+// the six fields amjdp and push gather, at the four corners of one cell
+template <typename R> struct Fields6 {
+    Corners<R> ex, ey, ez, bx, by, bz;
+};
+
+template <typename R>
+__device__ __forceinline__ Fields6<R> load_fields6(const R *fxy, const R *bxy, const R *psit,
+                                                   size_t cs, int ld, int cx, int cy)
+{
+    return Fields6<R>{load_corners(fxy, ld, cx, cy),      load_corners(fxy + cs, ld, cx, cy),
+                      load_corners(psit, ld, cx, cy),     load_corners(bxy, ld, cx, cy),
+                      load_corners(bxy + cs, ld, cx, cy), load_corners(bxy + 2 * cs, ld, cx, cy)};
+}
+
+// One particle of PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768), in the
+// reference's operation order: gather, half Boris step with the B_z
+// rotation, the 8 deposited quantities times the 4 corner weights added to
+// acc; returns the new p6 (part2d_lib77.f:597).
+template <typename R>
+__device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y, R px, R py,
+                                       R p6, R p7, R qm, R qtmh, R qbm, R dti, R dex, R idex,
+                                       R (&acc)[8][4])
+{
+    R dxp = x - fx, dyp = y - fy;
+    R amx = R(1) - dxp, amy = R(1) - dyp;
+    R inv_part_7 = recip(p7);
+    R qtmh1 = qtmh * inv_part_7;
+    R qtmh2 = qtmh1 * p6;
+    R dx = gather(F.ex, amx, amy, dxp, dyp);
+    R dy = gather(F.ey, amx, amy, dxp, dyp);
+    R dz = gather(F.ez, amx, amy, dxp, dyp);
+    R ox = gather(F.bx, amx, amy, dxp, dyp);
+    R oy = gather(F.by, amx, amy, dxp, dyp);
+    R oz = gather(F.bz, amx, amy, dxp, dyp);
+    R ddx = (R(-1)) * qtmh2 * dx + qtmh * oy;
+    R ddy = (R(-1)) * qtmh2 * dy - qtmh * ox;
+    R vx = px, vy = py;
+    R acx = vx + ddx, acy = vy + ddy;
+    R omzt = qtmh1 * oz;
+    R omt = omzt * omzt;
+    R anorm = R(2) * recip(R(1) + omt);
+    R rot1 = R(0.5) * (R(1) - omt);
+    R rot2 = omzt;
+    R v1 = (rot1 * acx + rot2 * acy) * anorm + ddx;
+    R v2 = (rot1 * acy - rot2 * acx) * anorm + ddy;
+    R qm1 = qm * inv_part_7;
+    amx = qm1 * amx;
+    dxp = qm1 * dxp;
+    ox = R(0.5) * (v1 + vx);
+    oy = R(0.5) * (v2 + vy);
+    oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
+    const R p6n = p7 + oz;
+    oz = oz * idex;
+    vx = (v1 - vx) * dti;
+    vy = (v2 - vy) * dti;
+    dz = qbm * (dz + (dx * ox + dy * oy) * dex * dex * inv_part_7);
+    const R w[4] = {amx * amy, dxp * amy, amx * dyp, dxp * dyp};
+    v1 = ox * ox * inv_part_7;
+    v2 = oy * oy * inv_part_7;
+    R v3 = ox * oy * inv_part_7;
+    vx = vx + ox * dz * inv_part_7;
+    vy = vy + oy * dz * inv_part_7;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        acc[0][k] += ox * w[k];
+        acc[1][k] += oy * w[k];
+        acc[2][k] += oz * w[k];
+        acc[3][k] += vx * w[k];
+        acc[4][k] += vy * w[k];
+        acc[5][k] += v1 * w[k];
+        acc[6][k] += v2 * w[k];
+        acc[7][k] += v3 * w[k];
+    }
+    return p6n;
+}
+
 // PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768): half Boris step with the B_z
 // rotation, deposit of J (3), dJ/dxi (2) and momentum flux (3); p6 updated.
 template <typename R, int RG, int CH, int WPE>
@@ -320,12 +396,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         const int i0 = off[key], i1 = off[key + 1];
         const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
-        const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
-        const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
-        const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
-        const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
-        const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
-        const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
+        const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
 #pragma unroll
@@ -351,57 +422,8 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
             const int b = min(i1, c0 + n);
             for (int i = a; i < b; i += T) {
                 const int l = lpad(i - c0);
-                R p6 = sp6[l], p7 = sp7[l], qm = sq[l];
-                R dxp = sx[l] - fx, dyp = sy[l] - fy;
-                R amx = R(1) - dxp, amy = R(1) - dyp;
-                R inv_part_7 = recip(p7);
-                R qtmh1 = qtmh * inv_part_7;
-                R qtmh2 = qtmh1 * p6;
-                R dx = gather(ex, amx, amy, dxp, dyp);
-                R dy = gather(ey, amx, amy, dxp, dyp);
-                R dz = gather(ez, amx, amy, dxp, dyp);
-                R ox = gather(bx, amx, amy, dxp, dyp);
-                R oy = gather(by, amx, amy, dxp, dyp);
-                R oz = gather(bz, amx, amy, dxp, dyp);
-                R ddx = (R(-1)) * qtmh2 * dx + qtmh * oy;
-                R ddy = (R(-1)) * qtmh2 * dy - qtmh * ox;
-                R vx = spx[l], vy = spy[l];
-                R acx = vx + ddx, acy = vy + ddy;
-                R omzt = qtmh1 * oz;
-                R omt = omzt * omzt;
-                R anorm = R(2) * recip(R(1) + omt);
-                R rot1 = R(0.5) * (R(1) - omt);
-                R rot2 = omzt;
-                R v1 = (rot1 * acx + rot2 * acy) * anorm + ddx;
-                R v2 = (rot1 * acy - rot2 * acx) * anorm + ddy;
-                R qm1 = qm * inv_part_7;
-                amx = qm1 * amx;
-                dxp = qm1 * dxp;
-                ox = R(0.5) * (v1 + vx);
-                oy = R(0.5) * (v2 + vy);
-                oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
-                sp6[l] = p7 + oz;
-                oz = oz * idex;
-                vx = (v1 - vx) * dti;
-                vy = (v2 - vy) * dti;
-                dz = qbm * (dz + (dx * ox + dy * oy) * dex * dex * inv_part_7);
-                const R w[4] = {amx * amy, dxp * amy, amx * dyp, dxp * dyp};
-                v1 = ox * ox * inv_part_7;
-                v2 = oy * oy * inv_part_7;
-                R v3 = ox * oy * inv_part_7;
-                vx = vx + ox * dz * inv_part_7;
-                vy = vy + oy * dz * inv_part_7;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    acc[0][k] += ox * w[k];
-                    acc[1][k] += oy * w[k];
-                    acc[2][k] += oz * w[k];
-                    acc[3][k] += vx * w[k];
-                    acc[4][k] += vy * w[k];
-                    acc[5][k] += v1 * w[k];
-                    acc[6][k] += v2 * w[k];
-                    acc[7][k] += v3 * w[k];
-                }
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], sp6[l], sp7[l],
+                                   sq[l], qtmh, qbm, dti, dex, idex, acc);
             }
             __syncthreads();
             for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
@@ -426,6 +448,113 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
 #pragma unroll
     for (int c = 0; c < 8; ++c)
         emit_nodes(nacc + c * 289, out, c, g.ldr, halo + c * halo_cs, tile, tx, ty);
+}
+
+// Orders LDS accesses among the lanes of one wave (no workgroup barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The same deposit with the waves of a tile working independently: wave w
+// takes the tile's row pairs w and w+4 (32 cells, two lanes per cell),
+// stages that pair's particles through its own LDS area in chunks of CHW,
+// and adds its cell sums into its own 3 x 17 node rows in four corner phases
+// (wave-level ordering only).  One workgroup barrier at the end, then each
+// tile node is the sum, in row-pair order, of the (one or two) row-pair node
+// rows that hold it.  A heavy row pair delays only its own wave.
+template <typename R, int CHW>
+__global__ void __launch_bounds__(256)
+k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
+          const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
+          const int *skip)
+{
+    if (skip && *skip) return;
+    constexpr int NP = Stage<CHW>::N, NR = 3 * 17;   // node rows of a row pair
+    extern __shared__ unsigned char smem_raw[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    R *base = reinterpret_cast<R *>(smem_raw);
+    R *sx = base + wave * 7 * NP, *sy = sx + NP, *spx = sy + NP, *spy = spx + NP;
+    R *sp6 = spy + NP, *sp7 = sp6 + NP, *sq = sp7 + NP;
+    R *rnode = base + 4 * 7 * NP;                     // [row pair 8][comp 8][3][17]
+    const int tile = blockIdx.x, tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
+    for (int n = tid; n < 8 * 8 * NR; n += TCELLS) rnode[n] = R(0);
+    __syncthreads();
+    const int cl = lane >> 1, sub = lane & 1;
+    const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
+    for (int it = 0; it < 2; ++it) {
+        const int rp = wave + 4 * it;
+        const int lrow = 2 * rp + (cl >> 4), lcol = cl & 15;
+        const int key = kb + lrow * TILE + lcol;
+        const int i0 = off[key], i1 = off[key + 1];
+        const int G0 = off[kb + 2 * rp * TILE], G1 = off[kb + (2 * rp + 2) * TILE];
+        const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
+        const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        const R fx = (R)cx, fy = (R)cy;
+        R acc[8][4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
+        for (int c0 = G0; c0 < G1; c0 += CHW) {
+            const int n = min(CHW, G1 - c0);
+            wave_sync();
+            for (int p = lane; p < n; p += 64) {
+                const int l = lpad(p), q = c0 + p;
+                sx[l] = s.x[q];
+                sy[l] = s.y[q];
+                spx[l] = s.px[q];
+                spy[l] = s.py[q];
+                sp6[l] = s.p6[q];
+                sp7[l] = s.p7[q];
+                sq[l] = s.q[q];
+            }
+            wave_sync();
+            int a = i0 + sub;
+            if (a < c0) a += ((c0 - a + 1) / 2) * 2;
+            const int b = min(i1, c0 + n);
+            for (int i = a; i < b; i += 2) {
+                const int l = lpad(i - c0);
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], sp6[l], sp7[l],
+                                   sq[l], qtmh, qbm, dti, dex, idex, acc);
+            }
+            wave_sync();
+            for (int p = lane; p < n; p += 64) s.p6[c0 + p] = sp6[lpad(p)];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] += __shfl_xor(acc[c][k], 1);
+        R *rn = rnode + rp * 8 * NR;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            wave_sync();
+            if (sub == 0) {
+                const int node = ((cl >> 4) + (k >> 1)) * 17 + lcol + (k & 1);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) rn[c * NR + node] += acc[c][k];
+            }
+        }
+    }
+    __syncthreads();
+    for (int n = tid; n < 17 * 17; n += TCELLS) {
+        const int i = n % 17, j = n / 17;
+        const int rp = j >> 1;                 // row pair whose node row j%2 is node row j
+        const bool top = (j & 1) == 0 && j > 0;   // also row 2 of row pair rp-1
+        for (int c = 0; c < 8; ++c) {
+            R v;
+            if (j == 16) v = rnode[(7 * 8 + c) * NR + 2 * 17 + i];
+            else {
+                v = rnode[(rp * 8 + c) * NR + (j & 1) * 17 + i];
+                if (top) v = rnode[((rp - 1) * 8 + c) * NR + 2 * 17 + i] + v;
+            }
+            if (i >= 1 && i < TILE && j >= 1 && j < TILE)
+                dep_store(out, c, (size_t)(ty * TILE + j) * g.ldr + tx * TILE + i, v);
+            else
+                halo[c * halo_cs + (size_t)tile * HALO + halo_index(i, j)] = v;
+        }
+    }
 }
 
 // PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
@@ -759,17 +888,22 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
 // Variant 0 of each table is the default; qp_ctx_tune selects another.
 struct DepShape {
     int rg, ch, wpe;
+    int wave;          // 1: k_amjdp_w (independent waves, staging chunk ch per wave)
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
-    static constexpr DepShape q[] = {{16, 2048, 1}, {16, 4096, 1}, {8, 2048, 1}, {16, 1024, 1}};
-    static constexpr DepShape j[] = {{8, 1024, 1}, {8, 2048, 1}, {4, 1024, 4}, {16, 1024, 4},
-                                     {4, 512, 4}, {8, 512, 1}};
+    static constexpr DepShape q[] = {{16, 2048, 1, 0}, {16, 4096, 1, 0}, {8, 2048, 1, 0},
+                                     {16, 1024, 1, 0}};
+    static constexpr DepShape j[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {4, 1024, 4, 0},
+                                     {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
+                                     {2, 256, 1, 1},   {2, 128, 1, 1}};
 };
 template <> struct DepTab<double> {
-    static constexpr DepShape q[] = {{8, 1024, 1}, {8, 2048, 1}, {16, 1024, 1}, {8, 512, 1}};
-    static constexpr DepShape j[] = {{8, 1024, 1}, {4, 1024, 1}, {4, 512, 1}, {8, 512, 1},
-                                     {2, 512, 1}, {16, 512, 1}};
+    static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
+                                     {8, 512, 1, 0}};
+    static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
+                                     {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
+                                     {2, 128, 1, 1},  {2, 64, 1, 1}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -914,8 +1048,16 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     const size_t pl = c->rplane;
     const size_t hcs = (size_t)c->ntiles * HALO;
     constexpr DepShape d = DepTab<R>::j[V];
-    const size_t lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
-    auto kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
+    size_t lds;
+    void (*kern)(TileGeom, Soa<R>, const int *, const R *, const R *, const R *, size_t,
+                 DepOut<R>, R *, size_t, R, R, R, const int *);
+    if constexpr (d.wave) {
+        lds = (4 * 7 * (size_t)Stage<d.ch>::N + 8 * 8 * 51) * sizeof(R);
+        kern = k_amjdp_w<R, d.ch>;
+    } else {
+        lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
+        kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
+    }
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
     {
@@ -940,6 +1082,8 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 3: rc = amjdp_v<R, 3>(sp, ef, bf, psit, dex, out); break;
     case 4: rc = amjdp_v<R, 4>(sp, ef, bf, psit, dex, out); break;
     case 5: rc = amjdp_v<R, 5>(sp, ef, bf, psit, dex, out); break;
+    case 6: rc = amjdp_v<R, 6>(sp, ef, bf, psit, dex, out); break;
+    case 7: rc = amjdp_v<R, 7>(sp, ef, bf, psit, dex, out); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out); break;
     }
     if (rc) return rc;

@@ -30,6 +30,8 @@ struct qp_sim {
     int iter;
     qp_field2d *f[QP_F_COUNT];
     qp_field3d *bexyz, *bbxyz;
+    qp_field2d *epw_hist;   // B_perp of the slice before the last (extrapolated seed)
+    int hist_slices;        // slices of this step recorded in the history
 };
 
 using namespace qp;
@@ -55,6 +57,7 @@ int qp_sim_create(qp_ctx *c, qp_species *const *sp, int nsp, qp_beam *const *bm,
     for (int i = 0; i < QP_F_COUNT && !rc; ++i) rc = qp_field2d_create(c, dims[i], st[i], &s->f[i]);
     if (!rc) rc = qp_field3d_create(c, 3, &s->bexyz);
     if (!rc) rc = qp_field3d_create(c, 3, &s->bbxyz);
+    if (!rc) rc = qp_field2d_create(c, 2, 1, &s->epw_hist);
     if (rc) { qp_sim_destroy(s); return rc; }
     *out = s;
     return QP_OK;
@@ -66,6 +69,7 @@ int qp_sim_destroy(qp_sim *s)
     for (int i = 0; i < QP_F_COUNT; ++i) qp_field2d_destroy(s->f[i]);
     qp_field3d_destroy(s->bexyz);
     qp_field3d_destroy(s->bbxyz);
+    qp_field2d_destroy(s->epw_hist);
     delete s;
     return QP_OK;
 }
@@ -104,6 +108,7 @@ int qp_sim_begin_step(qp_sim *s)
     cu->state = 1;
     RC(launch_fill(c, epw->ks.p, epw->dim * c->kplane, 0.0));
     epw->state = 1;
+    s->hist_slices = 0;
     RC(qp_field2d_as(s->f[QP_F_FXY], 0.0));
     RC(qp_field2d_as(s->f[QP_F_PSIT], 0.0));
     RC(qp_field2d_as(s->f[QP_F_BXYZ], 0.0));
@@ -168,6 +173,11 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     const int inv_kinds[2] = {1, 2};
     qp_field2d *fwd[3] = {cu, dcu, amu};
     const int fwd_kinds[3] = {1, 1, 3};
+    const bool adaptive = tol > 0.0;
+    if (adaptive && c->tune[QP_TUNE_SEED]) {
+        RC(fused_seed(epw, s->epw_hist, s->hist_slices >= 2));
+        ++s->hist_slices;
+    }
     RC(fused_pass(pf, dex, false, 0, 0.0));
     RC(fft_k2r_batch(c, inv, inv_kinds, 2));
     psit->state = bxyz->state = 0;
@@ -194,7 +204,6 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     };
     int l = 0;
     double res = -1.0;
-    const bool adaptive = tol > 0.0;
     if (!adaptive) {
         for (l = 1; l <= iter_max; ++l) RC(pass(l));
         l = iter_max;

@@ -150,3 +150,35 @@ def test_fp32_run_to_run_identical():
     b = _fp32_run(12)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
+def test_every_deposit_shape_matches_oracle(knob):
+    """Every compiled launch shape of the deposits gives the oracle's slice
+    (FP64, 1e-10), so an autotuner may pick any of them."""
+    banner(f"deposit shapes {knob}")
+    grid = qpo.Grid(IND, IND, INDZ)
+    bspec = qpo.BeamSpec(center=(0.0, 0.0, 1.5), sigma=(0.8, 0.8, 0.5), np_=(32, 32, 64),
+                         peak=4.0)
+    osim = qpo.Simulation(grid, [qpo.SpeciesSpec(ppc=(2, 2))], [bspec], dt=10.0, iter_=2)
+    osim.begin_step()
+    for j in range(1, 4):
+        osim.slice(j)
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP64)
+    _, nv = ctx.tune_get(getattr(qps, knob))
+    ctx.close()
+    for v in range(nv):
+        ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP64)
+        ctx.tune(getattr(qps, knob), v)
+        sp = qps.Species(ctx, ppc=(2, 2))
+        bm = qps.Beam(ctx, center=bspec.center, sigma=bspec.sigma, np_=bspec.np_,
+                      peak=bspec.peak, dt=10.0)
+        gsim = qps.Sim(ctx, [sp], [bm], iter_=2)
+        gsim.begin_step()
+        for j in range(1, 4):
+            gsim.slice(j)
+        for which, want in ((qps.F_PSIT, osim.psit), (qps.F_BXYZ, osim.bxyz),
+                            (qps.F_FXY, osim.fxy)):
+            err = rel(gsim.field(which).download(0), want)
+            assert err < 1e-10, (knob, v, which, err)
+        ctx.close()
