@@ -139,6 +139,17 @@ def run_gpu(args, precision, beam_part=None):
             sweep[f"seed[{v}] passes"] = float(np.mean(its[2:]))
             j0 += 10
         ctx.tune(qps.TUNE_SEED, -1)
+        # transforms: transpose kernels / column-pair y pass
+        for v in (0, 1):
+            ctx.tune(qps.TUNE_FFT, v)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for j in range(j0, j0 + 5):
+                sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+            ctx.sync()
+            sweep[f"fft[{v}] ms/slice"] = round((time.perf_counter() - t0) / 5 * 1e3, 4)
+            j0 += 5
+        ctx.tune(qps.TUNE_FFT, -1)
         log(f"[bench] sweep (ms per launch): {sweep}")
     # every kernel class: time, launches and algorithmic bytes (operands read
     # or written once) counted by the library at each launch
@@ -177,7 +188,7 @@ def run_gpu(args, precision, beam_part=None):
     }
     # finish the 3-D step (untimed): the remaining slices and the beam push;
     # sample the wake at a few slices for the FP32-vs-FP64 comparison
-    j0 = j_first + args.steps + nprof + (60 if args.sweep else 0)
+    j0 = j_first + args.steps + nprof + (70 if args.sweep else 0)
     for j in range(j0, ctx.nz + 1):
         sim.slice(j, iter_max=args.iter_max, tol=args.tol)
     g0 = bm.moments()[0]

@@ -105,9 +105,12 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * with a residual tolerance by linear extrapolation 2 B(xi-1) - B(xi-2)
  * instead of the reference's B(xi-1) (default 0); it changes only how many
  * passes reach the tolerance, and fixed-pass runs (tol <= 0) ignore it.
+ * QP_TUNE_FFT = 1 folds the transpose between the x and y passes of every
+ * transform into the y pass (column-pair reads / writes) instead of a
+ * separate LDS-tiled transpose (default 0); results are identical.
  * value -1 restores a knob's default. */
 enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_SEED = 3,
-       QP_TUNE_COUNT };
+       QP_TUNE_FFT = 4, QP_TUNE_COUNT };
 int qp_ctx_tune(qp_ctx *ctx, int knob, int value);
 /* current value and the number of values of a knob */
 int qp_ctx_tune_get(const qp_ctx *ctx, int knob, int *value, int *nvalues);

@@ -91,7 +91,13 @@ template <typename R> struct CompSet {
 // contiguous axis; row pair p of component blockIdx.y is transform p.  src
 // may equal dst (a block reads its rows completely before writing).  If
 // guard_row >= 0 the result of row 0 is also written to row guard_row.
-template <typename R, int L>
+// COL = 1: the transform rows are the column pairs of a row-major source
+// (element e of row r at src[e * ld_src + r]); COL = 2: the results are
+// written as column pairs of a row-major destination (dst[k * ld_dst + r]).
+// Either way a lane moves both rows of its pair with one 2-element access,
+// and block b takes pair (b mod 8) * (gridDim.x / 8) + b / 8, so the blocks
+// on one XCD (dispatched round-robin by block id) share cache lines.
+template <typename R, int L, int COL>
 __global__ void __launch_bounds__(FftShape<L>::BS)
 k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
            const cplx<R> *__restrict__ W, int guard_row, const int *skip, ResJob rj)
@@ -104,7 +110,8 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     const int comp = blockIdx.y;
     const int type = (types >> comp) & 1;   // 0 sine, 1 cosine
     const int f = threadIdx.x / TF, t = threadIdx.x % TF;
-    const int ra = 2 * (blockIdx.x * S::FB + f);
+    const int bx = COL ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int ra = 2 * (bx * S::FB + f);
     const bool live = ra < nrows, hb = ra + 1 < nrows;
     cplx<R> *buf = reinterpret_cast<cplx<R> *>(smem_raw) + f * S::LDSM;
 
@@ -139,14 +146,21 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     // first pass: radix 8 on the extension, straight from memory
     cplx<R> v[8];
     {
-        const R *sa = cs.src[comp] + (size_t)ra * ld_src;
+        const R *sa = cs.src[comp] + (COL == 1 ? (size_t)ra : (size_t)ra * ld_src);
         const R *sb = sa + ld_src;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int idx = t + r * TF;
             const int e = idx <= n ? idx : M - idx;
             R va = R(0), vb = R(0);
-            if (live) {
+            if (COL == 1) {
+                if (live) {
+                    // the pair of columns ra, ra+1 of row e (ra even, ld_src even)
+                    const cplx<R> p2 = *reinterpret_cast<const cplx<R> *>(sa + (size_t)e * ld_src);
+                    va = p2.x;
+                    vb = hb ? p2.y : R(0);
+                }
+            } else if (live) {
                 va = sa[e];
                 if (hb) vb = sb[e];
             }
@@ -160,7 +174,7 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     }
 
     const R half = scale * R(0.5);
-    R *da = cs.dst[comp] + (size_t)ra * ld_dst;
+    R *da = cs.dst[comp] + (COL == 2 ? (size_t)ra : (size_t)ra * ld_dst);
     R *db = da + ld_dst;
     R *dg = (guard_row >= 0 && ra == 0) ? cs.dst[comp] + (size_t)guard_row * ld_dst : nullptr;
     auto emit = [&](int k, cplx<R> z) {
@@ -176,9 +190,15 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
             oa = -z.y * half;
             ob = z.x * half;
         }
-        da[k] = oa;
-        if (hb) db[k] = ob;
-        if (dg) dg[k] = oa;
+        if (COL == 2) {
+            R *d = da + (size_t)k * ld_dst;
+            if (hb) *reinterpret_cast<cplx<R> *>(d) = cplx<R>{oa, ob};
+            else d[0] = oa;
+        } else {
+            da[k] = oa;
+            if (hb) db[k] = ob;
+            if (dg) dg[k] = oa;
+        }
     };
 
     if (S::NR8 == 1 && S::REM == 0) {   // M = 8 is below the supported sizes
@@ -288,7 +308,7 @@ void kind_types(int kind, int dim, int *tx, int *ty)
 
 static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
-template <typename R, int L>
+template <typename R, int L, int COL>
 static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int ld_src,
                        int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
@@ -296,16 +316,19 @@ static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, in
     const size_t lds = (size_t)S::FB * S::LDSM * sizeof(cplx<R>);
     QP_REQUIRE(lds <= c->max_lds, QP_ERR_UNSUPPORTED,
                "transform of length %d needs %zu B of LDS (limit %zu)", S::N, lds, c->max_lds);
-    auto kern = k_fft_rows<R, L>;
+    QP_REQUIRE(!COL || guard_row < 0, QP_ERR_ARG, "column transforms take no guard row");
+    auto kern = k_fft_rows<R, L, COL>;
     if (lds > 65536)
         QP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
     const int pairs = (nrows + 1) / 2;
-    dim3 grid((pairs + S::FB - 1) / S::FB, ncomp);
-    {
-    ProfScope _ps(c, QP_PROF_FFT, 2.0 * ncomp * nrows * (S::N + 1.0) * sizeof(R));
+    int gx = (pairs + S::FB - 1) / S::FB;
+    if (COL) gx = (gx + 7) & ~7;   // whole rounds of the 8 XCDs
+    dim3 grid(gx, ncomp);
     const ResJob rj = c->res_job;
     c->res_job = ResJob{};
+    {
+    ProfScope _ps(c, QP_PROF_FFT, 2.0 * ncomp * nrows * (S::N + 1.0) * sizeof(R));
     hipLaunchKernelGGL(kern, grid, dim3(S::BS), lds, c->stream, cs, nrows, ld_src, ld_dst, types,
                        (R)scale, W, guard_row, c->skip, rj);
     }
@@ -313,20 +336,20 @@ static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, in
     return QP_OK;
 }
 
-template <typename R>
+template <typename R, int COL>
 static int rowfft(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int n, int ld_src,
                   int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
     switch (ilog2(2 * n)) {
-    case 5: return rows_launch<R, 5>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 6: return rows_launch<R, 6>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 7: return rows_launch<R, 7>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 8: return rows_launch<R, 8>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 9: return rows_launch<R, 9>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 10: return rows_launch<R, 10>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 11: return rows_launch<R, 11>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 12: return rows_launch<R, 12>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 13: return rows_launch<R, 13>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 5: return rows_launch<R, 5, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 6: return rows_launch<R, 6, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 7: return rows_launch<R, 7, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 8: return rows_launch<R, 8, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 9: return rows_launch<R, 9, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 10: return rows_launch<R, 10, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 11: return rows_launch<R, 11, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 12: return rows_launch<R, 12, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 13: return rows_launch<R, 13, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
     }
     set_error("transform length %d unsupported", n);
     return QP_ERR_UNSUPPORTED;
@@ -388,11 +411,15 @@ static int r2k_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
         y.dst[i] = ks[i];
     }
     const double ani = 1.0 / ((double)c->nx * (double)c->ny);
-    int rc = rowfft<R>(c, a, nc, c->ny + 1, c->nx, c->ldr, c->ldr, mx, ani,
-                       c->twx.as<cplx<R>>(), -1);
-    if (!rc) rc = transpose<R>(c, b, nc, c->ldr, c->ldk, c->ny + 1, c->nx + 1);
-    if (!rc) rc = rowfft<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
+    int rc = rowfft<R, 0>(c, a, nc, c->ny + 1, c->nx, c->ldr, c->ldr, mx, ani,
+                          c->twx.as<cplx<R>>(), -1);
+    if (rc) return rc;
+    if (c->tune[QP_TUNE_FFT] == 1)   // y pass straight from the column pairs of the x pass
+        return rowfft<R, 1>(c, b, nc, c->nx + 1, c->ny, c->ldr, c->ldk, my, 1.0,
                             c->twy.as<cplx<R>>(), -1);
+    rc = transpose<R>(c, b, nc, c->ldr, c->ldk, c->ny + 1, c->nx + 1);
+    if (!rc) rc = rowfft<R, 0>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
+                               c->twy.as<cplx<R>>(), -1);
     return rc;
 }
 
@@ -404,7 +431,7 @@ static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
     const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
     if (nc <= 0) return nc;
     R *sk = c->scr_k.as<R>();
-    CompSet<R> y{}, b{}, a{};
+    CompSet<R> y{}, b{}, a{}, yc{};
     for (int i = 0; i < nc; ++i) {
         y.src[i] = ks[i];
         y.dst[i] = sk + i * c->kplane;
@@ -412,13 +439,21 @@ static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
         b.dst[i] = rs[i];
         a.src[i] = rs[i];
         a.dst[i] = rs[i];
+        yc.src[i] = ks[i];
+        yc.dst[i] = rs[i];
     }
-    int rc = rowfft<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,
-                       c->twy.as<cplx<R>>(), -1);
-    if (!rc) rc = transpose<R>(c, b, nc, c->ldk, c->ldr, c->nx + 1, c->ny + 1);
+    int rc;
+    if (c->tune[QP_TUNE_FFT] == 1) {   // y pass written as column pairs of rs
+        rc = rowfft<R, 2>(c, yc, nc, c->nx + 1, c->ny, c->ldk, c->ldr, my, 2.0,
+                          c->twy.as<cplx<R>>(), -1);
+    } else {
+        rc = rowfft<R, 0>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,
+                          c->twy.as<cplx<R>>(), -1);
+        if (!rc) rc = transpose<R>(c, b, nc, c->ldk, c->ldr, c->nx + 1, c->ny + 1);
+    }
     // rows 0..ny-1; row ny receives row 0 (guard copy, PPNCGUARD2L nvp = 1)
-    if (!rc) rc = rowfft<R>(c, a, nc, c->ny, c->nx, c->ldr, c->ldr, mx, 2.0,
-                            c->twx.as<cplx<R>>(), c->ny);
+    if (!rc) rc = rowfft<R, 0>(c, a, nc, c->ny, c->nx, c->ldr, c->ldr, mx, 2.0,
+                               c->twx.as<cplx<R>>(), c->ny);
     return rc;
 }
 
