@@ -215,12 +215,15 @@ __global__ void k_beam_deposit(int64_t np, const double *__restrict__ x, const d
 
 // qdpcopy / copyadd of one xi plane (ufield3d_class.f03:460-490) from the
 // fixed-point charge
+// (assign: q2 is taken as zero, as after asc(0), so it is not read)
 template <typename R>
-__global__ void k_beam_slice_add(R *q2, const long long *q3, size_t n)
+__global__ void k_beam_slice_add(R *q2, const long long *q3, size_t n, int assign)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        q2[i] = q2[i] + (R)((double)q3[i] * (1.0 / BEAM_FIX));
+         i += (size_t)gridDim.x * blockDim.x) {
+        const R v = (R)((double)q3[i] * (1.0 / BEAM_FIX));
+        q2[i] = assign ? R(0) + v : q2[i] + v;
+    }
 }
 
 // PGBPUSH32L_QP (part3d_lib77.f:630-786) with ipbc = 1 and cofd = 0; flags
@@ -418,7 +421,7 @@ int qp_beam_qdp(qp_beam *b)
 }
 
 // qdpcopy_beam3d (beam3d_class.f03:136-153): q += beam q at plane `slice`
-int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice)
+static int slice_copy(qp_beam *b, qp_field2d *q, int slice, int assign)
 {
     BCHECK(b);
     QP_REQUIRE(q && q->dim >= 1, QP_ERR_ARG, "qdpcopy: bad field");
@@ -427,17 +430,31 @@ int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice)
     const long long *src = b->q3.as<long long>() + (size_t)(slice - 1) * c->rplane;
     size_t n = c->rplane;
     dim3 grid((unsigned)((n + 255) / 256));
+    {
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, (assign ? 12.0 : 16.0) * n);
     if (c->prec == QP_FP64)
         hipLaunchKernelGGL(k_beam_slice_add<double>, grid, dim3(256), 0, c->stream,
-                           q->rs.as<double>(), src, n);
+                           q->rs.as<double>(), src, n, assign);
     else
         hipLaunchKernelGGL(k_beam_slice_add<float>, grid, dim3(256), 0, c->stream,
-                           q->rs.as<float>(), src, n);
+                           q->rs.as<float>(), src, n, assign);
+    }
     QP_LAUNCH_CHECK();
     q->state = 0;
     q->gcells = 1;
     return QP_OK;
 }
+
+int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice) { return slice_copy(b, q, slice, 0); }
+
+}  // extern "C"
+
+namespace qp {
+// the slice loop's first beam charge copy: qb = 0 + beam slice (asc then copyadd)
+int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice) { return slice_copy(b, q, slice, 1); }
+}  // namespace qp
+
+extern "C" {
 
 int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf, double dex, double dez)
 {

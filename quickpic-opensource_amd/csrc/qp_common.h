@@ -231,8 +231,8 @@ int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol);
 int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
 int fused_seed(qp_field2d *epw, qp_field2d *hist, int extrap);
 int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
-int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
-                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex);
+int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz, qp_field2d *psi,
+                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd);
 
 // ---- kernel-class timing ----
 int prof_begin(qp_ctx *c, int kind, double bytes);   // returns a slot or -1

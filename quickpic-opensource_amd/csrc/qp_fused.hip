@@ -170,13 +170,16 @@ __global__ void k_efield(size_t n, size_t cs, R *fxy, const R *__restrict__ bxyz
 }
 
 // slice writes (simulation_class.f03:731-735): fxy *= dex, copied to
-// bexyz(1:2); psit to bexyz(3); bxyz(1:2) *= dex; bxyz to bbxyz(1:3)
+// bexyz(1:2); psit to bexyz(3); bxyz(1:2) *= dex; bxyz to bbxyz(1:3); and
+// the psi *= dex^2 of :670 (psi has no reader after extpsi in the slice)
 template <typename R>
 __global__ void k_slice_write(size_t n, size_t cs, R *fxy, const R *__restrict__ psit, R *bxyz,
-                              R *be1, R *be2, R *be3, R *bb1, R *bb2, R *bb3, R dex)
+                              R *psi, R *be1, R *be2, R *be3, R *bb1, R *bb2, R *bb3, R dex,
+                              R dd)
 {
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x) {
+        psi[e] = psi[e] * dd;
         const R f1 = fxy[e] * dex, f2 = fxy[cs + e] * dex;
         const R b1 = bxyz[e] * dex, b2 = bxyz[cs + e] * dex;
         fxy[e] = f1;
@@ -315,27 +318,28 @@ int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz)
 
 template <typename R>
 static int slice_write_t(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
-                         qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex)
+                         qp_field2d *psi, qp_field3d *bexyz, qp_field3d *bbxyz, int lpos,
+                         double dex, double dd)
 {
     qp_ctx *c = fxy->ctx;
     const size_t n = c->rplane, st = (size_t)(c->nz + 1) * n, z = (size_t)(lpos - 1) * n;
     R *be = bexyz->buf.as<R>() + z, *bb = bbxyz->buf.as<R>() + z;
     {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 16.0 * n * sizeof(R));
+    ProfScope _ps(c, QP_PROF_ELEMENTWISE, 18.0 * n * sizeof(R));
     hipLaunchKernelGGL(k_slice_write<R>, flat_grid(n), dim3(256), 0, c->stream, n, n,
-                       fxy->rs.as<R>(), psit->rs.as<R>(), bxyz->rs.as<R>(), be, be + st,
-                       be + 2 * st, bb, bb + st, bb + 2 * st, (R)dex);
+                       fxy->rs.as<R>(), psit->rs.as<R>(), bxyz->rs.as<R>(), psi->rs.as<R>(), be,
+                       be + st, be + 2 * st, bb, bb + st, bb + 2 * st, (R)dex, (R)dd);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
 }
 
-int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
-                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex)
+int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz, qp_field2d *psi,
+                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd)
 {
     return fxy->ctx->prec == QP_FP64
-               ? slice_write_t<double>(fxy, psit, bxyz, bexyz, bbxyz, lpos, dex)
-               : slice_write_t<float>(fxy, psit, bxyz, bexyz, bbxyz, lpos, dex);
+               ? slice_write_t<double>(fxy, psit, bxyz, psi, bexyz, bbxyz, lpos, dex, dd)
+               : slice_write_t<float>(fxy, psit, bxyz, psi, bexyz, bbxyz, lpos, dex, dd);
 }
 
 }  // namespace qp

@@ -17,6 +17,7 @@ int species_check(qp_species *sp);
 int species_check_post(qp_species *sp);
 int species_check_poll(qp_species *sp, bool wait);
 int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first);
+int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice);
 int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
                         double dex, bool first, bool last);
@@ -149,8 +150,9 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     // the move check of an earlier slice, if it has arrived
     for (qp_species *sp : s->sp) RC(species_check_poll(sp, false));
     // beam charge and its transverse field (:650-656)
-    RC(qp_field2d_as(qb, 0.0));
-    for (qp_beam *b : s->bm) RC(qp_beam_qdpcopy(b, qb, j + 1));
+    if (s->bm.empty()) RC(qp_field2d_as(qb, 0.0));
+    for (size_t i = 0; i < s->bm.size(); ++i)
+        RC(i == 0 ? beam_slice_assign(s->bm[0], qb, j + 1) : qp_beam_qdpcopy(s->bm[i], qb, j + 1));
     RC(qp_field2d_fftrk(qb, 1));
     RC(qp_field2d_elf(qb, epwb));
     // plasma charge, psi and E_perp (:657-670)
@@ -165,8 +167,7 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
         fxy->state = psi->state = 0;
     }
     for (qp_species *sp : s->sp) RC(qp_species_extpsi(sp, psi, dex));
-    const double dd = dex * dex;
-    RC(qp_field2d_mult(psi, psi, 0, nullptr, nullptr, &dd));
+    const double dd = dex * dex;   // psi *= dex^2 (:670) is applied with the slice writes
     // predictor fields from the carried current (:671-675) and the B assembly
     const PassFields pf{cu, dcu, amu, epw, epwb, psit, bxyz};
     qp_field2d *inv[2] = {psit, bxyz};
@@ -240,15 +241,17 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     }
     if (l > iter_max) l = iter_max;
     // species charge diagnostic of the last pass (cbq, :692-695): it reads
-    // only the species' own q and cu, which the later steps leave untouched
-    for (size_t i = 0; i < s->sp.size(); ++i) RC(qp_species_cbq(s->sp[i], s->q3s[i], j + 1));
+    // only the species' own q and cu, which the later steps leave untouched;
+    // its one product is the 3-D charge, so it runs for species that have one
+    for (size_t i = 0; i < s->sp.size(); ++i)
+        if (s->q3s[i]) RC(qp_species_cbq(s->sp[i], s->q3s[i], j + 1));
     // final E_perp and predictor current (:721-726); B is already assembled
     RC(fused_efield(fxy, bxyz));
     RC(fused_predict_cu(cu, dcu, dxi));
     // plasma push (:727-730)
     for (qp_species *sp : s->sp) RC(species_push_nocheck(sp, fxy, bxyz, psit, dex));
     // slice j+1 of the 3-D wake (:731-735)
-    RC(fused_slice_write(fxy, psit, bxyz, s->bexyz, s->bbxyz, j + 1, dex));
+    RC(fused_slice_write(fxy, psit, bxyz, psi, s->bexyz, s->bbxyz, j + 1, dex, dd));
     for (qp_species *sp : s->sp) RC(species_check_post(sp));
     if (iters_used) *iters_used = l;
     if (residual) *residual = adaptive ? res : -1.0;

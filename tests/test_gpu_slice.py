@@ -182,3 +182,28 @@ def test_every_deposit_shape_matches_oracle(knob):
             err = rel(gsim.field(which).download(0), want)
             assert err < 1e-10, (knob, v, which, err)
         ctx.close()
+
+
+def test_species_charge_diagnostic_cbq():
+    """cbq (species2d_class.f03:399-418): the smoothed species charge plus Jz
+    of the last pass, copied into the 3-D q array each slice (FP64, 1e-10)."""
+    banner("cbq")
+    grid = qpo.Grid(IND, IND, INDZ)
+    bspec = qpo.BeamSpec(center=(0.0, 0.0, 1.5), sigma=(0.8, 0.8, 0.5), np_=(32, 32, 64),
+                         peak=4.0)
+    osim = qpo.Simulation(grid, [qpo.SpeciesSpec(ppc=(2, 2))], [bspec], dt=10.0, iter_=2,
+                          keep_q3=True)
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP64)
+    sp = qps.Species(ctx, ppc=(2, 2))
+    bm = qps.Beam(ctx, center=bspec.center, sigma=bspec.sigma, np_=bspec.np_, peak=bspec.peak,
+                  dt=10.0)
+    gsim = qps.Sim(ctx, [sp], [bm], iter_=2)
+    q3 = qps.Field3d(ctx, 1)
+    gsim.species_q3(0, q3)
+    osim.begin_step()
+    gsim.begin_step()
+    for j in range(1, 5):
+        osim.slice(j)
+        gsim.slice(j)
+        assert rel(q3.plane(1, j + 1), osim.q3s[0][j]) < 1e-10, j
+    ctx.close()
