@@ -1,0 +1,79 @@
+/*
+ * qpnet.h -- C ABI of the learned corrector seed's kernels in libqpslice.so:
+ * the forward and backward passes of a U-Net of 3x3 convolutions on the
+ * MFMA units (BF16 operands, FP32 accumulation), its loss and its optimiser.
+ *
+ * No counterpart in the reference, which seeds each slice's B-perp corrector
+ * with the previous slice's field (simulation_class.f03:676-710).  These
+ * entry points serve a caller that trains and runs the network
+ * (quickpic-opensource_amd/qpnet.py); the network changes only how many
+ * corrector passes reach a residual tolerance, never the converged answer.
+ *
+ * Tensors are device pointers.  Activations are BF16 bit patterns (uint16_t)
+ * in NHWC order with a channel stride cs (so a concatenation is two writers
+ * of one buffer at different channel offsets: pass the offset pointer).
+ * 3x3 weights are BF16 [tap][cout][cin] with tap = 3 (dy+1) + (dx+1); biases,
+ * gradients of weights and losses are FP32.  Channel counts are multiples of
+ * 16; cout <= 192.  stream is a hipStream_t (0: the null stream).  Every call
+ * enqueues on the stream and returns 0 or a negative status.
+ */
+#ifndef QPNET_H
+#define QPNET_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out = relu?(conv3x3(x * (mask > 0 if mask), w) + bias) (+ out if accum) */
+int qp_net_conv3x3(void *stream, const uint16_t *x, const uint16_t *mask, int H, int W,
+                   int cs_in, int cin, const uint16_t *w, const float *bias, uint16_t *out,
+                   int cs_out, int cout, int relu, int accum);
+/* dw[tap][cout][cin] = sum_p dy'[p][co] x[p+off][ci], dy' = dy * (yout > 0);
+ * db[co] = sum_p dy'[p][co] (db may be null); part: nchunk * 9 * cout * cin
+ * floats of scratch; chunks are summed in order, so the result is
+ * independent of scheduling */
+int qp_net_conv3x3_wgrad(void *stream, const uint16_t *x, int cs_in, int cin,
+                         const uint16_t *dy, const uint16_t *yout, int cs_out, int cout, int H,
+                         int W, int nchunk, float *part, float *dw, float *db);
+/* wt[tap][cin][cout] = w[8 - tap][cout][cin] (kernel of the data gradient) */
+int qp_net_flip_weights(void *stream, const uint16_t *w, int cout, int cin, uint16_t *wt);
+/* 2x2 average pooling (H, W even) and its gradient (din = / += dout / 4) */
+int qp_net_pool2(void *stream, const uint16_t *in, int cs_in, uint16_t *out, int cs_out, int H,
+                 int W, int C);
+int qp_net_pool2_bwd(void *stream, const uint16_t *dout, int cs_out, uint16_t *din, int cs_in,
+                     int H, int W, int C, int accum);
+/* nearest 2x upsampling of an H x W map and its gradient */
+int qp_net_up2(void *stream, const uint16_t *in, int cs_in, uint16_t *out, int cs_out, int H,
+               int W, int C);
+int qp_net_up2_bwd(void *stream, const uint16_t *dout, int cs_out, uint16_t *din, int cs_in,
+                   int H, int W, int C);
+/* nc FP32 planes (plane elements apart, row pitch ld) times scale[c] -> BF16
+ * NHWC with cs channels, channels >= nc zero */
+int qp_net_pack_input(void *stream, const float *planes, int nc, size_t plane, int ld,
+                      const float *scale, int H, int W, uint16_t *out, int cs);
+/* 1x1 output layer in FP32: o[c][p] = b[c] + sum_k w[c][k] h[p][k] */
+int qp_net_head(void *stream, const uint16_t *h, int cs, int cin, const float *w,
+                const float *b, int cout, size_t npix, float *o);
+/* its gradient: dh (BF16), dwb = [dw (cout x cin) | db (cout)] per output row
+ * c as (cin + 1) floats; part: nchunk * cout * (cin + 1) floats */
+int qp_net_head_bwd(void *stream, const uint16_t *h, int cs, int cin, const float *w, int cout,
+                    size_t npix, const float *dout, uint16_t *dh, int nchunk, float *part,
+                    float *dwb);
+/* mean squared error: grad = 2 (o - t) / n, per-block FP64 partial sums of
+ * (o - t)^2 in partial[0..nblk) */
+int qp_net_mse(void *stream, const float *o, const float *t, size_t n, float *grad,
+               double *partial, int nblk);
+/* Adam step (step >= 1) on FP32 master weights; wb (optional) gets their
+ * BF16 copy */
+int qp_net_adam(void *stream, float *wm, const float *g, float *m, float *v, size_t n, float lr,
+                float b1, float b2, float eps, int step, uint16_t *wb);
+int qp_net_to_bf16(void *stream, const float *a, size_t n, uint16_t *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

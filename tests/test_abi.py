@@ -42,3 +42,12 @@ def test_python_mirror_names_match_the_reference_methods():
         assert hasattr(qps.Species, m), m
     for m in ("qdp", "qdpcopy", "push"):
         assert hasattr(qps.Beam, m), m
+
+
+def test_library_exports_the_network_entry_points():
+    header = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "include", "qpnet.h")
+    declared = qps.exported_symbols(header)
+    assert len(declared) >= 13
+    missing = [s for s in declared if not hasattr(qps.lib(), s)]
+    assert not missing, missing

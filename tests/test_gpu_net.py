@@ -1,0 +1,189 @@
+"""The learned-seed network's BF16-MFMA kernels against the numpy oracle
+(oracle/qpnet_ref.py) on the same BF16 inputs.
+
+Bars: a kernel that accumulates exact BF16 products in FP32 and stores FP32
+(weight, bias and head gradients) agrees to 1e-5 relative; one that stores
+BF16 agrees to the BF16 rounding of its output (2^-8 relative) plus FP32
+accumulation.  The whole network's gradients are checked against the oracle
+composed layer by layer with the same BF16 roundings.
+"""
+import numpy as np
+import pytest
+
+from oracle import qpnet_ref as nr
+
+torch = pytest.importorskip("torch")
+qpnet = pytest.importorskip("qpnet")
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(name):
+    print(f"[gpu-test] {name}", flush=True)
+
+
+def bf(a):
+    """numpy float array rounded to BF16 (round to nearest even)."""
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32))
+    return t.to(torch.bfloat16).to(torch.float32).numpy().astype(np.float64)
+
+
+def dev_bits(a):
+    return qpnet.bf16_bits(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()).flatten()
+
+
+def host(bits, shape):
+    return qpnet.bits_to_f32(bits).cpu().numpy().astype(np.float64).reshape(shape)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("H,W,cin,cout", [(32, 32, 48, 32), (8, 80, 16, 192), (16, 16, 192, 64)])
+def test_conv3x3_forward_and_gradients(H, W, cin, cout):
+    banner(f"conv3x3 {H}x{W} {cin}->{cout}")
+    rng = np.random.default_rng(H + cin + cout)
+    x = bf(rng.standard_normal((H, W, cin)))
+    w = bf(rng.standard_normal((9, cout, cin)) * 0.1)
+    b = rng.standard_normal(cout).astype(np.float32).astype(np.float64) * 0.1
+    want = nr.conv3x3(x, w, b)
+    xd, wd = dev_bits(x), dev_bits(w)
+    bd = torch.from_numpy(b.astype(np.float32)).cuda()
+    yd = torch.zeros(H * W * cout, dtype=torch.int16, device="cuda")
+    qpnet.conv3x3(xd, H, W, cin, cin, wd, bd, yd, cout, cout)
+    y = host(yd, (H, W, cout))
+    assert rel(y, want) < 8e-3
+    # gradients through the ReLU, with the GPU's own BF16 output as y
+    gy = bf(rng.standard_normal((H, W, cout)))
+    gyd = dev_bits(gy)
+    gx_want, gw_want, gb_want = nr.conv3x3_bwd(x, w, y, gy)
+    part = torch.zeros(32 * 9 * cout * cin, device="cuda")
+    gw = torch.zeros(9 * cout * cin, device="cuda")
+    gb = torch.zeros(cout, device="cuda")
+    qpnet.conv3x3_wgrad(xd, cin, cin, gyd, yd, cout, cout, H, W, part, gw, gb)
+    assert rel(gw.cpu().numpy().reshape(9, cout, cin), gw_want) < 1e-5
+    assert rel(gb.cpu().numpy(), gb_want) < 1e-5
+    wt = torch.zeros_like(wd)
+    qpnet.flip_weights(wd, cout, cin, wt)
+    gxd = torch.zeros(H * W * cin, dtype=torch.int16, device="cuda")
+    qpnet.conv3x3(gyd, H, W, cout, cout, wt, None, gxd, cin, cin, relu=False, mask=yd)
+    assert rel(host(gxd, (H, W, cin)), gx_want) < 8e-3
+
+
+def test_pool_up_head():
+    banner("pool / up / head")
+    L = qpnet._lib()
+    rng = np.random.default_rng(5)
+    H, W, C = 8, 12, 16
+    x = bf(rng.standard_normal((H, W, C)))
+    xd = dev_bits(x)
+    pd = torch.zeros(H * W * C // 4, dtype=torch.int16, device="cuda")
+    qpnet._call(L.qp_net_pool2, qpnet.stream(), qpnet.ptr(xd), C, qpnet.ptr(pd), C, H, W, C)
+    assert rel(host(pd, (H // 2, W // 2, C)), nr.pool2(x)) < 4e-3
+    g = bf(rng.standard_normal((H // 2, W // 2, C)))
+    gd = dev_bits(g)
+    bd = torch.zeros(H * W * C, dtype=torch.int16, device="cuda")
+    qpnet._call(L.qp_net_pool2_bwd, qpnet.stream(), qpnet.ptr(gd), C, qpnet.ptr(bd), C, H, W, C, 0)
+    assert np.array_equal(host(bd, (H, W, C)), nr.pool2_bwd(g))   # exact: times 1/4
+    ud = torch.zeros(H * W * C, dtype=torch.int16, device="cuda")
+    qpnet._call(L.qp_net_up2, qpnet.stream(), qpnet.ptr(gd), C, qpnet.ptr(ud), C, H // 2, W // 2, C)
+    assert np.array_equal(host(ud, (H, W, C)), nr.up2(g))
+    ubd = torch.zeros(H * W * C // 4, dtype=torch.int16, device="cuda")
+    qpnet._call(L.qp_net_up2_bwd, qpnet.stream(), qpnet.ptr(xd), C, qpnet.ptr(ubd), C, H // 2, W // 2, C)
+    assert rel(host(ubd, (H // 2, W // 2, C)), nr.up2_bwd(x)) < 4e-3
+    hw = rng.standard_normal((2, C)).astype(np.float32)
+    hb = rng.standard_normal(2).astype(np.float32)
+    o = torch.zeros(2 * H * W, device="cuda")
+    qpnet._call(L.qp_net_head, qpnet.stream(), qpnet.ptr(xd), C, C,
+                qpnet.ptr(torch.from_numpy(hw).cuda()), qpnet.ptr(torch.from_numpy(hb).cuda()), 2,
+                H * W, qpnet.ptr(o))
+    assert rel(o.cpu().numpy().reshape(2, -1), nr.head(x, hw, hb)) < 1e-5
+
+
+def _numpy_unet(net, x):
+    """The network composed from the oracle's layers with the GPU's BF16
+    roundings; returns the activations and the output."""
+    c1, c2, c3, c4 = qpnet.CH
+    W = lambda cv: bf(cv.w.cpu().numpy())   # noqa: E731
+    B = lambda cv: cv.b.cpu().numpy().astype(np.float64)   # noqa: E731
+    e, d = net.enc, net.dec
+    A = {}
+    A["x0"] = x
+    A["e1a"] = bf(nr.conv3x3(x, W(e[0]), B(e[0])))
+    A["e1"] = bf(nr.conv3x3(A["e1a"], W(e[1]), B(e[1])))
+    A["p1"] = bf(nr.pool2(A["e1"]))
+    A["e2a"] = bf(nr.conv3x3(A["p1"], W(e[2]), B(e[2])))
+    A["e2"] = bf(nr.conv3x3(A["e2a"], W(e[3]), B(e[3])))
+    A["p2"] = bf(nr.pool2(A["e2"]))
+    A["e3a"] = bf(nr.conv3x3(A["p2"], W(e[4]), B(e[4])))
+    A["e3"] = bf(nr.conv3x3(A["e3a"], W(e[5]), B(e[5])))
+    A["p3"] = bf(nr.pool2(A["e3"]))
+    A["ba"] = bf(nr.conv3x3(A["p3"], W(e[6]), B(e[6])))
+    A["bb"] = bf(nr.conv3x3(A["ba"], W(e[7]), B(e[7])))
+    A["cat3"] = np.concatenate([nr.up2(A["bb"]), A["e3"]], axis=2)
+    A["d3a"] = bf(nr.conv3x3(A["cat3"], W(d[0]), B(d[0])))
+    A["d3b"] = bf(nr.conv3x3(A["d3a"], W(d[1]), B(d[1])))
+    A["cat2"] = np.concatenate([nr.up2(A["d3b"]), A["e2"]], axis=2)
+    A["d2a"] = bf(nr.conv3x3(A["cat2"], W(d[2]), B(d[2])))
+    A["d2b"] = bf(nr.conv3x3(A["d2a"], W(d[3]), B(d[3])))
+    A["cat1"] = np.concatenate([nr.up2(A["d2b"]), A["e1"]], axis=2)
+    A["d1a"] = bf(nr.conv3x3(A["cat1"], W(d[4]), B(d[4])))
+    A["d1b"] = bf(nr.conv3x3(A["d1a"], W(d[5]), B(d[5])))
+    out = nr.head(A["d1b"], net.hw.cpu().numpy().astype(np.float64),
+                  net.hb.cpu().numpy().astype(np.float64))
+    return A, out
+
+
+def test_unet_forward_backward_against_oracle():
+    banner("unet forward/backward")
+    H = W = 32
+    rng = np.random.default_rng(9)
+    net = qpnet.UNet(seed=3)
+    for cv in net.enc + net.dec:   # non-zero biases so every ReLU branch is taken
+        cv.b.copy_(torch.from_numpy(rng.standard_normal(cv.cout).astype(np.float32) * 0.05))
+    planes = rng.standard_normal((qpnet.NIN, H, W)).astype(np.float32)
+    pd = torch.from_numpy(planes).cuda().flatten()
+    out = net.forward(pd, H * W, W, H, W).cpu().numpy().astype(np.float64)
+    x = np.zeros((H, W, 16))
+    x[:, :, :qpnet.NIN] = bf(np.transpose(planes, (1, 2, 0)))
+    A, want = _numpy_unet(net, x)
+    assert rel(out, want) < 5e-2
+    tgt = rng.standard_normal(out.shape).astype(np.float32)
+    loss = net.backward(torch.from_numpy(tgt).cuda())
+    torch.cuda.synchronize()
+    ref_loss, gout = nr.mse(out, tgt.astype(np.float64))
+    assert abs(loss / ref_loss - 1) < 1e-6
+    # the last two layers' gradients from the GPU's own activations
+    c1 = qpnet.CH[0]
+    hw = net.hw.cpu().numpy().astype(np.float64)
+    d1b = host(net.a["d1b"], (H, W, c1))
+    g_d1b = (gout.T @ hw).reshape(H, W, c1)
+    gw_head = net.ghwb.cpu().numpy().reshape(2, c1 + 1)
+    assert rel(gw_head[:, :c1], gout @ d1b.reshape(-1, c1)) < 1e-5
+    d1a = host(net.a["d1a"], (H, W, c1))
+    _, gw5, gb5 = nr.conv3x3_bwd(d1a, bf(net.dec[5].w.cpu().numpy()), d1b, bf(g_d1b))
+    assert rel(net.dec[5].gw.cpu().numpy(), gw5) < 2e-2
+    assert rel(net.dec[5].gb.cpu().numpy(), gb5) < 2e-2
+    # every parameter received a finite, non-zero gradient
+    for cv in net.enc + net.dec:
+        g = cv.gw.cpu().numpy()
+        assert np.isfinite(g).all() and np.abs(g).max() > 0
+
+
+def test_training_reduces_the_loss():
+    banner("unet training")
+    H = W = 32
+    rng = np.random.default_rng(11)
+    net = qpnet.UNet(seed=4)
+    planes = torch.from_numpy(rng.standard_normal((qpnet.NIN, H, W)).astype(np.float32)).cuda()
+    yy, xx = np.mgrid[0:H, 0:W] / H
+    tgt = np.stack([np.sin(3 * xx) * np.cos(2 * yy), np.cos(4 * yy) * xx]).astype(np.float32)
+    tgt_d = torch.from_numpy(tgt.reshape(2, -1)).cuda()
+    losses = []
+    for _ in range(60):
+        net.forward(planes.flatten(), H * W, W, H, W)
+        losses.append(net.backward(tgt_d))
+        net.adam(lr=3e-3)
+    print("loss", losses[0], "->", losses[-1], flush=True)
+    assert losses[-1] < 0.3 * losses[0]
