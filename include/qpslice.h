@@ -103,8 +103,10 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * corrector pass l+1 before reading the residual of pass l (default 0: wait
  * for each residual).  QP_TUNE_SEED = 1 seeds the corrector of a slice run
  * with a residual tolerance by linear extrapolation 2 B(xi-1) - B(xi-2)
- * instead of the reference's B(xi-1) (default 0); it changes only how many
- * passes reach the tolerance, and fixed-pass runs (tol <= 0) ignore it.
+ * instead of the reference's B(xi-1) (default 0); QP_TUNE_SEED = 2 keeps the
+ * B_perp a caller wrote between qp_slice_prepare and qp_slice_finish.  The
+ * seed changes only how many passes reach the tolerance; fixed-pass runs
+ * (tol <= 0) use the reference's seed whatever the knob.
  * QP_TUNE_FFT = 1 folds the transpose between the x and y passes of every
  * transform into the y pass (column-pair reads / writes) instead of a
  * separate LDS-tiled transpose (default 0); results are identical.
@@ -127,6 +129,13 @@ int qp_field2d_upload(qp_field2d *f, int space, const double *host, int ld1,
                       int ld2);
 int qp_field2d_download(const qp_field2d *f, int space, double *host, int ld1,
                         int ld2);
+/* device storage of one space: element (c, row, col) at
+ * ptr + c*comp_stride + row*ld (elements of the context's precision; real:
+ * rows y = 0..ny, cols x = 0..nx; k: rows kx, cols ky).  For callers that
+ * compute on the fields in place (the learned seed); no reference
+ * counterpart. */
+int qp_field2d_device(const qp_field2d *f, int space, void **ptr, int *ld,
+                      size_t *comp_stride);
 int qp_field2d_as(qp_field2d *f, double value);                    /* asc :493 */
 int qp_field2d_asa(qp_field2d *f, const qp_field2d *that);         /* asa :624 */
 int qp_field2d_fftrk(qp_field2d *f, int kind);                     /* :157-191 */
@@ -269,6 +278,15 @@ int qp_sim_begin_step(qp_sim *sim);
  * the passes used and the last relative change (FP64). */
 int qp_slice_run(qp_sim *sim, int j, int iter_max, double tol,
                  int *iters_used, double *residual);
+/* qp_slice_run in two parts, for a caller that sets the corrector's seed
+ * itself (the learned seed): prepare runs the slice up to the seed (beam and
+ * plasma charge, psi, E_perp, the psi gather, :650-670); finish runs the
+ * rest.  Between them the caller may overwrite B_perp (QP_F_EPW, k space)
+ * through qp_field2d_device; with QP_TUNE_SEED = 2 finish keeps it as the
+ * seed.  No counterpart in the reference. */
+int qp_slice_prepare(qp_sim *sim, int j);
+int qp_slice_finish(qp_sim *sim, int j, int iter_max, double tol, int *iters_used,
+                    double *residual);
 /* end of the 3-D step: beam push (:752-757) and species renew (:761-764) */
 int qp_sim_end_step(qp_sim *sim, double s);
 

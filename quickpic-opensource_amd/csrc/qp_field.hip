@@ -539,6 +539,16 @@ int qp_field2d_download(const qp_field2d *f, int space, double *host, int ld1, i
                               : download_planes<float>(c, dev, host, f->dim, rows, cols, ld, cs, ld1);
 }
 
+int qp_field2d_device(const qp_field2d *f, int space, void **ptr, int *ld, size_t *comp_stride)
+{
+    FCHECK(f);
+    QP_REQUIRE(space == 0 || space == 1, QP_ERR_ARG, "qp_field2d_device: space %d", space);
+    if (ptr) *ptr = space == 0 ? f->rs.p : f->ks.p;
+    if (ld) *ld = space == 0 ? f->ctx->ldr : f->ctx->ldk;
+    if (comp_stride) *comp_stride = space == 0 ? f->ctx->rplane : f->ctx->kplane;
+    return QP_OK;
+}
+
 // asc (field2d_class.f03:493-510): real space set to value, state 0
 int qp_field2d_as(qp_field2d *f, double value)
 {

@@ -33,6 +33,7 @@ struct qp_sim {
     qp_field3d *bexyz, *bbxyz;
     qp_field2d *epw_hist;   // B_perp of the slice before the last (extrapolated seed)
     int hist_slices;        // slices of this step recorded in the history
+    int prepared;           // slice whose first part has run (0: none)
 };
 
 using namespace qp;
@@ -135,17 +136,18 @@ int qp_sim_begin_step(qp_sim *s)
 //    runs at the end of this one, beside psit's;
 //  * -fxy + B, the predictor cu -= dxi dcu and the slice writes are one
 //    kernel each.
-int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, double *residual)
+// first part of a slice: beam and plasma charge, psi, E_perp, the psi
+// gather (:650-670), up to the corrector seed
+int qp_slice_prepare(qp_sim *s, int j)
 {
-    QP_REQUIRE(s, QP_ERR_ARG, "qp_slice_run: null sim");
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_slice_prepare: null sim");
     qp_ctx *c = s->ctx;
-    QP_REQUIRE(j >= 1 && j <= c->nz, QP_ERR_ARG, "qp_slice_run: slice %d outside 1..%d", j, c->nz);
-    QP_REQUIRE(iter_max >= 1, QP_ERR_ARG, "qp_slice_run: iter_max %d", iter_max);
-    const double dex = c->dex, dxi = c->dxi;
+    QP_REQUIRE(j >= 1 && j <= c->nz, QP_ERR_ARG, "qp_slice_prepare: slice %d outside 1..%d", j,
+               c->nz);
+    const double dex = c->dex;
     qp_field2d **F = s->f;
     qp_field2d *qb = F[QP_F_QB], *qe = F[QP_F_QE], *psi = F[QP_F_PSI], *fxy = F[QP_F_FXY];
-    qp_field2d *cu = F[QP_F_CU], *dcu = F[QP_F_DCU], *amu = F[QP_F_AMU], *bxyz = F[QP_F_BXYZ];
-    qp_field2d *psit = F[QP_F_PSIT], *epw = F[QP_F_EPW], *epwb = F[QP_F_EPWB];
+    qp_field2d *epwb = F[QP_F_EPWB];
     const int nsp = (int)s->sp.size();
     // the move check of an earlier slice, if it has arrived
     for (qp_species *sp : s->sp) RC(species_check_poll(sp, false));
@@ -167,6 +169,24 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
         fxy->state = psi->state = 0;
     }
     for (qp_species *sp : s->sp) RC(qp_species_extpsi(sp, psi, dex));
+    s->prepared = j;
+    return QP_OK;
+}
+
+// the rest of the slice: corrector seed, predictor, passes, push, slice writes
+int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used, double *residual)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_slice_finish: null sim");
+    qp_ctx *c = s->ctx;
+    QP_REQUIRE(s->prepared == j, QP_ERR_STATE, "qp_slice_finish: slice %d was not prepared", j);
+    QP_REQUIRE(iter_max >= 1, QP_ERR_ARG, "qp_slice_finish: iter_max %d", iter_max);
+    s->prepared = 0;
+    const double dex = c->dex, dxi = c->dxi;
+    qp_field2d **F = s->f;
+    qp_field2d *psi = F[QP_F_PSI], *fxy = F[QP_F_FXY];
+    qp_field2d *cu = F[QP_F_CU], *dcu = F[QP_F_DCU], *amu = F[QP_F_AMU], *bxyz = F[QP_F_BXYZ];
+    qp_field2d *psit = F[QP_F_PSIT], *epw = F[QP_F_EPW], *epwb = F[QP_F_EPWB];
+    const int nsp = (int)s->sp.size();
     const double dd = dex * dex;   // psi *= dex^2 (:670) is applied with the slice writes
     // predictor fields from the carried current (:671-675) and the B assembly
     const PassFields pf{cu, dcu, amu, epw, epwb, psit, bxyz};
@@ -175,7 +195,7 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     qp_field2d *fwd[3] = {cu, dcu, amu};
     const int fwd_kinds[3] = {1, 1, 3};
     const bool adaptive = tol > 0.0;
-    if (adaptive && c->tune[QP_TUNE_SEED]) {
+    if (adaptive && c->tune[QP_TUNE_SEED] == 1) {
         RC(fused_seed(epw, s->epw_hist, s->hist_slices >= 2));
         ++s->hist_slices;
     }
@@ -256,6 +276,12 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     if (iters_used) *iters_used = l;
     if (residual) *residual = adaptive ? res : -1.0;
     return QP_OK;
+}
+
+int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, double *residual)
+{
+    RC(qp_slice_prepare(s, j));
+    return qp_slice_finish(s, j, iter_max, tol, iters_used, residual);
 }
 
 int qp_sim_end_step(qp_sim *s, double snew)

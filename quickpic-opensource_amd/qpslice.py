@@ -184,6 +184,13 @@ class Field2d:
         ld2 = a.shape[0]
         _call(lib().qp_field2d_upload, self.h, space, _ptr(a), ld1, ld2)
 
+    def device(self, space):
+        """(device address, row pitch, component stride) of one space, in
+        elements of the context's precision."""
+        p, ld, cs = C.c_void_p(), C.c_int(), C.c_size_t()
+        _call(lib().qp_field2d_device, self.h, space, C.byref(p), C.byref(ld), C.byref(cs))
+        return p.value, ld.value, cs.value
+
     def download(self, space=None):
         c = self.ctx
         space = self.state if space is None else space
@@ -485,6 +492,17 @@ class Sim:
     def slice(self, j, iter_max=None, tol=0.0):
         it, res = C.c_int(), C.c_double()
         _call(lib().qp_slice_run, self.h, j, self.iter if iter_max is None else iter_max,
+              _D(tol), C.byref(it), C.byref(res))
+        return it.value, res.value
+
+    def prepare(self, j):
+        """First part of slice j, up to the corrector seed."""
+        _call(lib().qp_slice_prepare, self.h, j)
+
+    def finish(self, j, iter_max=None, tol=0.0):
+        """The rest of slice j after prepare; returns (passes, residual)."""
+        it, res = C.c_int(), C.c_double()
+        _call(lib().qp_slice_finish, self.h, j, self.iter if iter_max is None else iter_max,
               _D(tol), C.byref(it), C.byref(res))
         return it.value, res.value
 
