@@ -75,6 +75,10 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6],
                   int precision, int device, qp_ctx **out);
 int qp_ctx_destroy(qp_ctx *ctx);
 int qp_ctx_sync(qp_ctx *ctx);
+/* The context's HIP stream (a hipStream_t), on which every call of this
+ * library enqueues; callers that compute on fields in place (through
+ * qp_field2d_device) enqueue on it too.  No reference counterpart. */
+int qp_ctx_stream(const qp_ctx *ctx, void **stream);
 int qp_ctx_info(const qp_ctx *ctx, int *nx, int *ny, int *nz, int *precision,
                 double *dex, double *dxi);
 const char *qp_last_error(void);

@@ -302,6 +302,13 @@ int qp_ctx_sync(qp_ctx *c)
     return QP_OK;
 }
 
+int qp_ctx_stream(const qp_ctx *c, void **stream)
+{
+    QP_REQUIRE(c && stream, QP_ERR_ARG, "qp_ctx_stream: null argument");
+    *stream = (void *)c->stream;
+    return QP_OK;
+}
+
 int qp_ctx_info(const qp_ctx *c, int *nx, int *ny, int *nz, int *precision, double *dex,
                 double *dxi)
 {

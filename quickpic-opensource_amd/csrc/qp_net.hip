@@ -294,6 +294,74 @@ __global__ void k_pack_input(const float *__restrict__ planes, int nc, size_t pl
     }
 }
 
+struct PlaneSet {
+    const void *p[8];
+    int ld[8];
+};
+
+__global__ void k_block_mean(PlaneSet ps, int nc, int dbl, int f, int H, int W, float *out)
+{
+    const size_t n = (size_t)nc * H * W;
+    const float inv = 1.f / (float)(f * f);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int X = (int)(i % W);
+        const size_t r = i / W;
+        const int Y = (int)(r % H), c = (int)(r / H);
+        const size_t o = (size_t)f * Y * ps.ld[c] + (size_t)f * X;
+        float s = 0.f;
+        if ((dbl >> c) & 1) {
+            const double *q = (const double *)ps.p[c] + o;
+            for (int a = 0; a < f; ++a)
+                for (int b = 0; b < f; ++b) s += (float)q[(size_t)a * ps.ld[c] + b];
+        } else {
+            const float *q = (const float *)ps.p[c] + o;
+            for (int a = 0; a < f; ++a)
+                for (int b = 0; b < f; ++b) s += q[(size_t)a * ps.ld[c] + b];
+        }
+        out[i] = s * inv;
+    }
+}
+
+// bilinear weights of fine node t (0 <= t < f n) between coarse centres
+// f T + (f - 1) / 2, clamped at the first and last centre
+__device__ __forceinline__ void lin1(int t, int f, int n, int &i0, int &i1, float &w1)
+{
+    const float u = ((float)t - 0.5f * (float)(f - 1)) / (float)f;
+    if (u <= 0.f) { i0 = i1 = 0; w1 = 0.f; return; }
+    if (u >= (float)(n - 1)) { i0 = i1 = n - 1; w1 = 0.f; return; }
+    i0 = (int)u;
+    i1 = i0 + 1;
+    w1 = u - (float)i0;
+}
+
+template <typename R>
+__global__ void k_seed_apply(const float *__restrict__ corr, int H, int W, int f, float s,
+                             const float *__restrict__ a, const float *__restrict__ b, int ld_ab,
+                             size_t pl_ab, R *dst, int ld_d, size_t pl_d, int ny1, int nx1)
+{
+    const size_t n = (size_t)2 * ny1 * nx1;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % nx1);
+        const size_t r = i / nx1;
+        const int y = (int)(r % ny1), c = (int)(r / ny1);
+        const size_t ab = c * pl_ab + (size_t)y * ld_ab + x;
+        float v = 2.f * a[ab] - b[ab];
+        if (y < f * H && x < f * W) {
+            int y0, y1, x0, x1;
+            float wy, wx;
+            lin1(y, f, H, y0, y1, wy);
+            lin1(x, f, W, x0, x1, wx);
+            const float *q = corr + (size_t)c * H * W;
+            const float top = (1.f - wx) * q[(size_t)y0 * W + x0] + wx * q[(size_t)y0 * W + x1];
+            const float bot = (1.f - wx) * q[(size_t)y1 * W + x0] + wx * q[(size_t)y1 * W + x1];
+            v += s * ((1.f - wy) * top + wy * bot);
+        }
+        dst[c * pl_d + (size_t)y * ld_d + x] = (R)v;
+    }
+}
+
 // output layer (1x1, cin -> cout <= 4) in FP32: o[c][p] = b[c] + sum_k w[c][k] h[p][k]
 __global__ void k_head(const uint16_t *__restrict__ h, int cs, int cin, const float *__restrict__ w,
                        const float *__restrict__ b, int cout, size_t npix, float *o)
@@ -491,6 +559,44 @@ int qp_net_pack_input(void *stream_, const float *planes, int nc, size_t plane, 
     hipStream_t st = (hipStream_t)stream_;
     hipLaunchKernelGGL(k_pack_input, g1((size_t)H * W * cs), dim3(256), 0, st, planes, nc, plane,
                        ld, scale, H, W, out, cs);
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int qp_net_block_mean(void *stream_, int nc, const void *const *planes, const int *ld, int dbl,
+                      int f, int H, int W, float *out)
+{
+    hipStream_t st = (hipStream_t)stream_;
+    NET_CHECK(nc >= 1 && nc <= 8 && planes && ld && out && f >= 1 && H > 0 && W > 0,
+              "qp_net_block_mean: bad argument");
+    PlaneSet ps{};
+    for (int c = 0; c < nc; ++c) {
+        NET_CHECK(planes[c] && ld[c] >= f * W, "qp_net_block_mean: plane %d", c);
+        ps.p[c] = planes[c];
+        ps.ld[c] = ld[c];
+    }
+    hipLaunchKernelGGL(k_block_mean, g1((size_t)nc * H * W), dim3(256), 0, st, ps, nc, dbl, f, H,
+                       W, out);
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int qp_net_seed_apply(void *stream_, const float *corr, int H, int W, int f, float s,
+                      const float *a, const float *b, int ld_ab, size_t pl_ab, void *dst,
+                      int dst_double, int ld_d, size_t pl_d, int ny1, int nx1)
+{
+    hipStream_t st = (hipStream_t)stream_;
+    NET_CHECK(corr && a && b && dst && H > 0 && W > 0 && f >= 1 && ny1 > 0 && nx1 > 0 &&
+                  ld_ab >= nx1 && ld_d >= nx1 && pl_ab >= (size_t)ld_ab * ny1 &&
+                  pl_d >= (size_t)ld_d * ny1,
+              "qp_net_seed_apply: bad argument");
+    const size_t n = (size_t)2 * ny1 * nx1;
+    if (dst_double)
+        hipLaunchKernelGGL(k_seed_apply<double>, g1(n), dim3(256), 0, st, corr, H, W, f, s, a, b,
+                           ld_ab, pl_ab, (double *)dst, ld_d, pl_d, ny1, nx1);
+    else
+        hipLaunchKernelGGL(k_seed_apply<float>, g1(n), dim3(256), 0, st, corr, H, W, f, s, a, b,
+                           ld_ab, pl_ab, (float *)dst, ld_d, pl_d, ny1, nx1);
     QP_LAUNCH_CHECK();
     return QP_OK;
 }

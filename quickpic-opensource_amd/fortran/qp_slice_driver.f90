@@ -41,7 +41,8 @@ program qp_slice_driver
   call bxyz%new(ctx, 3, 0); call cu%new(ctx, 3, 1); call dcu%new(ctx, 2, 0)
   call amu%new(ctx, 3, 0); call epw%new(ctx, 2, 1); call epwb%new(ctx, 2, 0)
   call bexyz%new(ctx, 3); call bbxyz%new(ctx, 3)
-  call sp%new(ctx, [2, 2], 1.0d0, -1.0d0, 1.0d0, 10.0d0, 0.0d0)
+  ! the species advances by the xi step dxi per slice (simulation_class.f03:578)
+  call sp%new(ctx, [2, 2], 1.0d0, -1.0d0, 1.0d0, dxi, 0.0d0)
   call bm%new(ctx, [0.0d0, 0.0d0, 1.0d0], [1.0d0, 1.0d0, 0.6d0], [1.0d0, 1.0d0, 0.0d0], &
               [16, 16, 32], 4.0d0, 2.0d4, 10.0d0)
 

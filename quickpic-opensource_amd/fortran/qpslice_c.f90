@@ -39,6 +39,11 @@ module qpslice_c
        import :: c_int, c_ptr
        type(c_ptr), value :: c
      end function qp_ctx_sync
+     integer(c_int) function qp_ctx_stream(c, stream) bind(C, name="qp_ctx_stream")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: c
+       type(c_ptr), intent(out) :: stream
+     end function qp_ctx_stream
      integer(c_int) function qp_ctx_info(c, nx, ny, nz, prec, dex, dxi) &
           bind(C, name="qp_ctx_info")
        import :: c_int, c_double, c_ptr

@@ -45,6 +45,9 @@ def _lib():
             "qp_net_mse": [vp, fp, fp, sz, fp, dp, i],
             "qp_net_adam": [vp, fp, fp, fp, fp, sz, f, f, f, f, i, u16p],
             "qp_net_to_bf16": [vp, fp, sz, u16p],
+            "qp_net_block_mean": [vp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_int), i, i, i, i,
+                                  fp],
+            "qp_net_seed_apply": [vp, fp, i, i, i, f, fp, fp, i, sz, vp, i, i, sz, i, i],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -67,6 +70,23 @@ def ptr(t, offset=0):
 
 def stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def block_mean(planes, f, H, W, out):
+    """planes: list of (device address, row pitch, is_fp64); out: FP32
+    tensor [len(planes)][H][W] of f x f block means."""
+    n = len(planes)
+    ps = (C.c_void_p * n)(*[p for p, _, _ in planes])
+    lds = (C.c_int * n)(*[ld for _, ld, _ in planes])
+    dbl = sum(1 << c for c, (_, _, d) in enumerate(planes) if d)
+    _call(_lib().qp_net_block_mean, stream(), n, ps, lds, dbl, f, H, W, ptr(out))
+
+
+def seed_apply(corr, H, W, f, s, hist, dst, dst_double, ld_d, pl_d, ny1, nx1):
+    """dst <- 2 hist[0] - hist[1] + s up(corr) (see qpnet.h); hist: FP32
+    tensor [2][2][ny1][nx1]; dst: device address of the 2-component plane."""
+    _call(_lib().qp_net_seed_apply, stream(), ptr(corr), H, W, f, s, ptr(hist[0]), ptr(hist[1]),
+          nx1, ny1 * nx1, C.c_void_p(dst), 1 if dst_double else 0, ld_d, pl_d, ny1, nx1)
 
 
 def bf16_bits(a: torch.Tensor) -> torch.Tensor:

@@ -1,0 +1,226 @@
+"""Learned corrector seed for the slice loop, and the training of its network.
+
+Between qp_slice_prepare and qp_slice_finish of slice j the network
+(qpnet.UNet) reads, in real space on the nx x ny nodes:
+  B_perp of slices j-1 and j-2 (4 channels), the plasma charge of slice j,
+  the current J_x, J_y of the slice before and psi of slice j,
+and predicts the correction of B_perp over 2 B(j-1) - B(j-2).  The corrected
+field, transformed to k space, becomes the corrector's seed (QP_TUNE_SEED =
+2); the plain iteration to the residual tolerance is unchanged, so the
+network only changes the number of passes.  B_perp here is the corrector's
+field epw (in real space through the kind-1 transform of two components).
+
+Every device operation here runs on the context's stream
+(torch.cuda.ExternalStream over qp_ctx_stream), so torch's element-wise
+steps, the network's kernels and the library's slice calls are ordered
+without host synchronisation.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import qpnet
+import qpslice as qps
+
+
+def _wrap(addr, n, dtype):
+    """A torch tensor over n elements of existing device memory."""
+    class _Holder:
+        pass
+    h = _Holder()
+    h.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f4" if dtype == torch.float32
+                                  else "<f8", "data": (addr, False), "version": 2}
+    return torch.as_tensor(h, device="cuda")
+
+
+def ctx_stream(ctx: qps.Ctx):
+    """A torch stream object for the context's HIP stream."""
+    return torch.cuda.ExternalStream(ctx.stream())
+
+
+class SeedFeatures:
+    """Features, targets and seeds of one simulation, on the device, on a
+    grid coarsened f times (the network's grid: (ny / f) x (nx / f), both
+    multiples of 8).  Call its methods inside
+    `with torch.cuda.stream(ctx_stream(ctx))`."""
+
+    def __init__(self, ctx: qps.Ctx, sim: qps.Sim, f=1):
+        self.ctx, self.sim, self.f = ctx, sim, f
+        self.nx, self.ny = ctx.nx, ctx.ny
+        self.H, self.W = self.ny // f, self.nx // f
+        if self.H * f != self.ny or self.W * f != self.nx or self.H % 8 or self.W % 8:
+            raise ValueError(f"grid {self.nx}x{self.ny} does not coarsen by {f} to multiples of 8")
+        self.fp64 = ctx.precision == qps.FP64
+        self.dtype = torch.float64 if self.fp64 else torch.float32
+        self.tmp = qps.Field2d(ctx, 2, state=1)
+        self.hist = torch.zeros((2, 2, self.ny + 1, self.nx + 1), dtype=torch.float32,
+                                device="cuda")
+        self.nhist = 0
+        self.feat = torch.zeros((qpnet.NIN, self.H, self.W), dtype=torch.float32, device="cuda")
+        self.targ = torch.zeros((2, self.H, self.W), dtype=torch.float32, device="cuda")
+        self.diff = torch.zeros((2, self.ny + 1, self.nx + 1), dtype=torch.float32,
+                                device="cuda")
+
+    def view(self, field, comp, space=0):
+        """Component comp of a field as a (ny+1, nx+1) view of its storage
+        (k space: (nx+1, ny+1))."""
+        p, ld, cs = field.device(space)
+        es = 8 if self.fp64 else 4
+        rows, cols = (self.ny + 1, self.nx + 1) if space == 0 else (self.nx + 1, self.ny + 1)
+        buf = _wrap(p + comp * cs * es, rows * ld, self.dtype)
+        return buf.view(rows, ld)[:, :cols]
+
+    def _real_plane(self, field, comp):
+        p, ld, cs = field.device(0)
+        return (p + comp * cs * (8 if self.fp64 else 4), ld, self.fp64)
+
+    def _hist_plane(self, t):
+        return (t.data_ptr(), self.nx + 1, False)
+
+    def gather(self):
+        """After qp_slice_prepare: the 8 input planes on the network's grid:
+        B_perp of the last two slices, this slice's plasma charge, the
+        previous slice's J_x, J_y and this slice's psi."""
+        h = self.hist
+        cu = self.sim.field(qps.F_CU)
+        planes = [self._hist_plane(h[0, 0]), self._hist_plane(h[0, 1]),
+                  self._hist_plane(h[1, 0]), self._hist_plane(h[1, 1]),
+                  self._real_plane(self.sim.field(qps.F_QE), 0),
+                  self._real_plane(cu, 0), self._real_plane(cu, 1),
+                  self._real_plane(self.sim.field(qps.F_PSI), 0)]
+        qpnet.block_mean(planes, self.f, self.H, self.W, self.feat)
+        return self.feat
+
+    def record(self, target=False):
+        """After qp_slice_finish: the converged B_perp (real space) enters
+        the history; with target, first the network's target, the block
+        means of converged minus extrapolated B_perp, into self.targ."""
+        self.tmp.asa(self.sim.field(qps.F_EPW))
+        self.tmp.fftkr(1)
+        e = [self.view(self.tmp, c) for c in range(2)]
+        if target:
+            for c in range(2):
+                torch.sub(e[c].float(), 2.0 * self.hist[0, c] - self.hist[1, c],
+                          out=self.diff[c])
+            qpnet.block_mean([self._hist_plane(self.diff[0]), self._hist_plane(self.diff[1])],
+                             self.f, self.H, self.W, self.targ)
+        self.hist[1].copy_(self.hist[0])
+        for c in range(2):
+            self.hist[0, c].copy_(e[c])
+        self.nhist += 1
+
+    def set_seed(self, corr, scale):
+        """epw <- kind-1 forward transform of 2 B(j-1) - B(j-2) + scale
+        up(corr): corr is the network's (2, H, W) output."""
+        p, ld, cs = self.tmp.device(0)
+        qpnet.seed_apply(corr, self.H, self.W, self.f, scale, self.hist, p, self.fp64, ld, cs,
+                         self.ny + 1, self.nx + 1)
+        self.tmp.fftrk(1)
+        self.sim.field(qps.F_EPW).asa(self.tmp)
+
+    def reset(self):
+        self.nhist = 0
+
+
+class LearnedSeed:
+    """Runs slices with the network's seed (QP_TUNE_SEED = 2)."""
+
+    def __init__(self, ctx: qps.Ctx, sim: qps.Sim, net: qpnet.UNet, f=1):
+        self.ctx, self.sim, self.net = ctx, sim, net
+        ctx.tune(qps.TUNE_SEED, 2)
+        self.stream = ctx_stream(ctx)
+        with torch.cuda.stream(self.stream):
+            self.fs = SeedFeatures(ctx, sim, f)
+
+    def slice(self, j, iter_max=20, tol=1e-6):
+        fs = self.fs
+        with torch.cuda.stream(self.stream):
+            self.sim.prepare(j)
+            if fs.nhist >= 2:
+                x = fs.gather()
+                out = self.net.forward(x, fs.H * fs.W, fs.W, fs.H, fs.W)
+                fs.set_seed(out, self.net.out_scale)
+            it, res = self.sim.finish(j, iter_max=iter_max, tol=tol)
+            fs.record()
+        return it, res
+
+    def begin_step(self):
+        self.sim.begin_step()
+        self.fs.reset()
+
+
+def collect(ctx, sim, j0, j1, f=1, iter_max=20, tol=1e-6, every=1, sink=None):
+    """Runs slices j0..j1-1 with the extrapolated seed (QP_TUNE_SEED = 1);
+    for every `every`-th slice with two slices of history, calls
+    sink(features, target) (device tensors, valid during the call).
+    Returns the passes of every slice."""
+    ctx.tune(qps.TUNE_SEED, 1)
+    passes = []
+    with torch.cuda.stream(ctx_stream(ctx)):
+        fs = SeedFeatures(ctx, sim, f)
+        for j in range(j0, j1):
+            sim.prepare(j)
+            take = sink is not None and fs.nhist >= 2 and (j - j0) % every == 0
+            if take:
+                fs.gather()
+            it, _ = sim.finish(j, iter_max=iter_max, tol=tol)
+            passes.append(it)
+            fs.record(target=take)
+            if take:
+                sink(fs.feat, fs.targ)
+    ctx.tune(qps.TUNE_SEED, -1)
+    return passes
+
+
+class Trainer:
+    """Online training of the network on (features, target) pairs as the
+    plain solver produces them.  The first `warm` pairs fix the per-channel
+    input scales and the output scale (root mean squares) and fill a replay
+    buffer; every later pair replaces a random buffer entry and is followed
+    by `per_pair` Adam steps on random entries."""
+
+    def __init__(self, net: qpnet.UNet, H, W, buf=64, warm=16, per_pair=1, lr=1e-3, seed=0):
+        self.net, self.H, self.W = net, H, W
+        self.x = torch.zeros((buf, qpnet.NIN, H, W), device="cuda")
+        self.y = torch.zeros((buf, 2, H, W), device="cuda")
+        self.n, self.warm, self.per_pair, self.lr = 0, warm, per_pair, lr
+        self.rng = np.random.default_rng(seed)
+        self.losses = []
+
+    def _fit_scales(self):
+        k = min(self.n, len(self.x))
+        xs = self.x[:k].transpose(0, 1).reshape(qpnet.NIN, -1)
+        rms = torch.sqrt((xs * xs).mean(dim=1))
+        self.net.in_scale.copy_(1.0 / torch.clamp(rms, min=1e-20))
+        self.net.out_scale = float(torch.sqrt((self.y[:k] ** 2).mean()).clamp(min=1e-20))
+
+    def add(self, x, y):
+        b = len(self.x)
+        i = self.n if self.n < b else int(self.rng.integers(b))
+        self.x[i].copy_(x)
+        self.y[i].copy_(y)
+        self.n += 1
+        if self.n == self.warm:
+            self._fit_scales()
+        if self.n >= self.warm:
+            for _ in range(self.per_pair):
+                self.step()
+
+    def step(self):
+        k = min(self.n, len(self.x))
+        i = int(self.rng.integers(k))
+        H, W = self.H, self.W
+        self.net.forward(self.x[i], H * W, W, H, W)
+        t = (self.y[i] / self.net.out_scale).reshape(2, -1)
+        self.losses.append(self.net.backward(t))
+        self.net.adam(lr=self.lr)
+
+
+def save(net, path):
+    np.savez_compressed(path, **{k: v.numpy() for k, v in net.state().items()})
+
+
+def load(net, path):
+    d = np.load(path, allow_pickle=False)
+    net.load({k: torch.from_numpy(d[k]) for k in d.files})

@@ -117,6 +117,13 @@ class Ctx:
     def sync(self):
         _call(lib().qp_ctx_sync, self.h)
 
+    def stream(self):
+        """The context's HIP stream handle (an int), on which the library
+        enqueues every call."""
+        h = C.c_void_p()
+        _call(lib().qp_ctx_stream, self.h, C.byref(h))
+        return h.value
+
     def profile(self, enable=True):
         """Starts (and resets) or stops kernel-class timing with HIP events."""
         _call(lib().qp_ctx_profile, self.h, 1 if enable else 0)

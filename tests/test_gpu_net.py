@@ -187,3 +187,63 @@ def test_training_reduces_the_loss():
         net.adam(lr=3e-3)
     print("loss", losses[0], "->", losses[-1], flush=True)
     assert losses[-1] < 0.3 * losses[0]
+
+
+def _bilinear_up(c, f, ny1, nx1):
+    """numpy restatement of the seed's upsampling: bilinear between coarse
+    pixel centres f*T + (f-1)/2, clamped at the edges, zero beyond f*H, f*W."""
+    H, W = c.shape
+    out = np.zeros((ny1, nx1))
+
+    def w(t, n):
+        u = (t - 0.5 * (f - 1)) / f
+        if u <= 0:
+            return 0, 0, 0.0
+        if u >= n - 1:
+            return n - 1, n - 1, 0.0
+        i0 = int(np.floor(u))
+        return i0, i0 + 1, u - i0
+
+    for y in range(min(ny1, f * H)):
+        y0, y1, wy = w(y, H)
+        for x in range(min(nx1, f * W)):
+            x0, x1, wx = w(x, W)
+            top = (1 - wx) * c[y0, x0] + wx * c[y0, x1]
+            bot = (1 - wx) * c[y1, x0] + wx * c[y1, x1]
+            out[y, x] = (1 - wy) * top + wy * bot
+    return out
+
+
+@pytest.mark.parametrize("f", [1, 2, 4])
+def test_block_mean_and_seed_apply(f):
+    banner(f"block mean / seed apply f={f}")
+    rng = np.random.default_rng(20 + f)
+    ny, nx = 32, 48
+    ld32, ld64 = nx + 7, nx + 3
+    a32 = rng.standard_normal((ny + 1, ld32)).astype(np.float32)
+    a64 = rng.standard_normal((ny + 1, ld64))
+    d32 = torch.from_numpy(a32).cuda()
+    d64 = torch.from_numpy(a64).cuda()
+    H, W = ny // f, nx // f
+    out = torch.zeros((2, H, W), device="cuda")
+    qpnet.block_mean([(d32.data_ptr(), ld32, False), (d64.data_ptr(), ld64, True)], f, H, W, out)
+    got = out.cpu().numpy()
+    for c, src in enumerate((a32.astype(np.float64), a64)):
+        want = src[: f * H, : f * W].reshape(H, f, W, f).mean(axis=(1, 3))
+        assert np.abs(got[c] - want).max() < 1e-5 * max(1.0, np.abs(want).max())
+    # seed: 2 a - b + s up(corr), into an FP64 and an FP32 destination
+    hist = torch.from_numpy(rng.standard_normal((2, 2, ny + 1, nx + 1)).astype(np.float32)).cuda()
+    corr = torch.from_numpy(rng.standard_normal((2, H, W)).astype(np.float32)).cuda()
+    s = 0.37
+    h = hist.cpu().numpy().astype(np.float64)
+    cc = corr.cpu().numpy().astype(np.float64)
+    want = np.stack([2 * h[0, c] - h[1, c] + s * _bilinear_up(cc[c], f, ny + 1, nx + 1)
+                     for c in range(2)])
+    for dbl, ldd in ((True, nx + 5), (False, nx + 9)):
+        pl = (ny + 1) * ldd + 11
+        dst = torch.zeros(2 * pl, dtype=torch.float64 if dbl else torch.float32, device="cuda")
+        qpnet.seed_apply(corr, H, W, f, s, hist, dst.data_ptr(), dbl, ldd, pl, ny + 1, nx + 1)
+        d = dst.cpu().numpy().astype(np.float64)
+        got = np.stack([d[c * pl: c * pl + (ny + 1) * ldd].reshape(ny + 1, ldd)[:, : nx + 1]
+                        for c in range(2)])
+        assert np.abs(got - want).max() < 2e-5 * np.abs(want).max()
