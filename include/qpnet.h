@@ -60,15 +60,21 @@ int qp_net_pack_input(void *stream, const float *planes, int nc, size_t plane, i
  * network runs on this coarse grid (f = 1: a plain copy). */
 int qp_net_block_mean(void *stream, int nc, const void *const *planes, const int *ld, int dbl,
                       int f, int H, int W, float *out);
-/* Corrector seed in real space on ny1 x nx1 nodes (2 components):
+/* The network's correction in real space on ny1 x nx1 nodes (2 components):
  * dst[c][y][x] = 2 a[c][y][x] - b[c][y][x] + s up(corr[c])(y, x), where up
  * interpolates the coarse H x W correction bilinearly between coarse pixel
  * centres (clamped at the edges) on the nodes y < f H, x < f W and is zero on
- * the others.  a, b: FP32 planes with pitch ld_ab, plane stride pl_ab; dst:
- * FP64 if dst_double else FP32, pitch ld_d, plane stride pl_d. */
+ * the others; a and b may both be NULL (taken as zero).  a, b: FP32 planes
+ * with pitch ld_ab, plane stride pl_ab; dst: FP64 if dst_double else FP32,
+ * pitch ld_d, plane stride pl_d. */
 int qp_net_seed_apply(void *stream, const float *corr, int H, int W, int f, float s,
                       const float *a, const float *b, int ld_ab, size_t pl_ab, void *dst,
                       int dst_double, int ld_d, size_t pl_d, int ny1, int nx1);
+/* Corrector seed in k space over n elements (FP64 if dbl, else FP32), the
+ * library's extrapolated seed plus a correction: e = epw; with extrap,
+ * epw = 2 e - hist (+ corr if not NULL); then hist = e. */
+int qp_net_seed_combine(void *stream, void *epw, void *hist, const void *corr, size_t n,
+                        int extrap, int dbl);
 /* 1x1 output layer in FP32: o[c][p] = b[c] + sum_k w[c][k] h[p][k] */
 int qp_net_head(void *stream, const uint16_t *h, int cs, int cin, const float *w,
                 const float *b, int cout, size_t npix, float *o);

@@ -347,7 +347,7 @@ __global__ void k_seed_apply(const float *__restrict__ corr, int H, int W, int f
         const size_t r = i / nx1;
         const int y = (int)(r % ny1), c = (int)(r / ny1);
         const size_t ab = c * pl_ab + (size_t)y * ld_ab + x;
-        float v = 2.f * a[ab] - b[ab];
+        float v = a ? 2.f * a[ab] - b[ab] : 0.f;
         if (y < f * H && x < f * W) {
             int y0, y1, x0, x1;
             float wy, wx;
@@ -359,6 +359,17 @@ __global__ void k_seed_apply(const float *__restrict__ corr, int H, int W, int f
             v += s * ((1.f - wy) * top + wy * bot);
         }
         dst[c * pl_d + (size_t)y * ld_d + x] = (R)v;
+    }
+}
+
+template <typename R>
+__global__ void k_seed_combine(R *epw, R *hist, const R *__restrict__ corr, size_t n, int extrap)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const R e = epw[i];
+        if (extrap) epw[i] = corr ? R(2) * e - hist[i] + corr[i] : R(2) * e - hist[i];
+        hist[i] = e;
     }
 }
 
@@ -586,9 +597,9 @@ int qp_net_seed_apply(void *stream_, const float *corr, int H, int W, int f, flo
                       int dst_double, int ld_d, size_t pl_d, int ny1, int nx1)
 {
     hipStream_t st = (hipStream_t)stream_;
-    NET_CHECK(corr && a && b && dst && H > 0 && W > 0 && f >= 1 && ny1 > 0 && nx1 > 0 &&
-                  ld_ab >= nx1 && ld_d >= nx1 && pl_ab >= (size_t)ld_ab * ny1 &&
-                  pl_d >= (size_t)ld_d * ny1,
+    NET_CHECK(corr && dst && H > 0 && W > 0 && f >= 1 && ny1 > 0 && nx1 > 0 && ld_d >= nx1 &&
+                  pl_d >= (size_t)ld_d * ny1 && (!a) == (!b) &&
+                  (!a || (ld_ab >= nx1 && pl_ab >= (size_t)ld_ab * ny1)),
               "qp_net_seed_apply: bad argument");
     const size_t n = (size_t)2 * ny1 * nx1;
     if (dst_double)
@@ -597,6 +608,21 @@ int qp_net_seed_apply(void *stream_, const float *corr, int H, int W, int f, flo
     else
         hipLaunchKernelGGL(k_seed_apply<float>, g1(n), dim3(256), 0, st, corr, H, W, f, s, a, b,
                            ld_ab, pl_ab, (float *)dst, ld_d, pl_d, ny1, nx1);
+    QP_LAUNCH_CHECK();
+    return QP_OK;
+}
+
+int qp_net_seed_combine(void *stream_, void *epw, void *hist, const void *corr, size_t n,
+                        int extrap, int dbl)
+{
+    hipStream_t st = (hipStream_t)stream_;
+    NET_CHECK(epw && hist && n > 0, "qp_net_seed_combine: bad argument");
+    if (dbl)
+        hipLaunchKernelGGL(k_seed_combine<double>, g1(n), dim3(256), 0, st, (double *)epw,
+                           (double *)hist, (const double *)corr, n, extrap);
+    else
+        hipLaunchKernelGGL(k_seed_combine<float>, g1(n), dim3(256), 0, st, (float *)epw,
+                           (float *)hist, (const float *)corr, n, extrap);
     QP_LAUNCH_CHECK();
     return QP_OK;
 }

@@ -48,6 +48,7 @@ def _lib():
             "qp_net_block_mean": [vp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_int), i, i, i, i,
                                   fp],
             "qp_net_seed_apply": [vp, fp, i, i, i, f, fp, fp, i, sz, vp, i, i, sz, i, i],
+            "qp_net_seed_combine": [vp, vp, vp, vp, sz, i, i],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -83,10 +84,19 @@ def block_mean(planes, f, H, W, out):
 
 
 def seed_apply(corr, H, W, f, s, hist, dst, dst_double, ld_d, pl_d, ny1, nx1):
-    """dst <- 2 hist[0] - hist[1] + s up(corr) (see qpnet.h); hist: FP32
-    tensor [2][2][ny1][nx1]; dst: device address of the 2-component plane."""
-    _call(_lib().qp_net_seed_apply, stream(), ptr(corr), H, W, f, s, ptr(hist[0]), ptr(hist[1]),
-          nx1, ny1 * nx1, C.c_void_p(dst), 1 if dst_double else 0, ld_d, pl_d, ny1, nx1)
+    """dst <- 2 hist[0] - hist[1] + s up(corr) (see qpnet.h), or s up(corr)
+    when hist is None; hist: FP32 tensor [2][2][ny1][nx1]; dst: device address
+    of the 2-component plane."""
+    a, b = (ptr(hist[0]), ptr(hist[1])) if hist is not None else (None, None)
+    _call(_lib().qp_net_seed_apply, stream(), ptr(corr), H, W, f, s, a, b, nx1, ny1 * nx1,
+          C.c_void_p(dst), 1 if dst_double else 0, ld_d, pl_d, ny1, nx1)
+
+
+def seed_combine(epw, hist, corr, n, extrap, dbl):
+    """k space: e = epw; epw = 2 e - hist (+ corr) if extrap; hist = e
+    (device addresses; corr may be None)."""
+    _call(_lib().qp_net_seed_combine, stream(), C.c_void_p(epw), C.c_void_p(hist),
+          C.c_void_p(corr) if corr else None, n, 1 if extrap else 0, 1 if dbl else 0)
 
 
 def bf16_bits(a: torch.Tensor) -> torch.Tensor:

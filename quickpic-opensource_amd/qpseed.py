@@ -4,11 +4,12 @@ Between qp_slice_prepare and qp_slice_finish of slice j the network
 (qpnet.UNet) reads, in real space on the nx x ny nodes:
   B_perp of slices j-1 and j-2 (4 channels), the plasma charge of slice j,
   the current J_x, J_y of the slice before and psi of slice j,
-and predicts the correction of B_perp over 2 B(j-1) - B(j-2).  The corrected
-field, transformed to k space, becomes the corrector's seed (QP_TUNE_SEED =
-2); the plain iteration to the residual tolerance is unchanged, so the
-network only changes the number of passes.  B_perp here is the corrector's
-field epw (in real space through the kind-1 transform of two components).
+and predicts the correction of B_perp over 2 B(j-1) - B(j-2).  The seed is
+that extrapolation, formed in k space as the library's own extrapolated seed
+is, plus the correction's kind-1 transform (QP_TUNE_SEED = 2); the plain
+iteration to the residual tolerance is unchanged, so the network only changes
+the number of passes.  B_perp here is the corrector's field epw (in real space
+through the kind-1 transform of two components).
 
 Every device operation here runs on the context's stream
 (torch.cuda.ExternalStream over qp_ctx_stream), so torch's element-wise
@@ -54,6 +55,8 @@ class SeedFeatures:
         self.fp64 = ctx.precision == qps.FP64
         self.dtype = torch.float64 if self.fp64 else torch.float32
         self.tmp = qps.Field2d(ctx, 2, state=1)
+        self.hk = qps.Field2d(ctx, 2, state=1)   # k-space B_perp of the slice before
+        self.nk = 0
         self.hist = torch.zeros((2, 2, self.ny + 1, self.nx + 1), dtype=torch.float32,
                                 device="cuda")
         self.nhist = 0
@@ -110,17 +113,28 @@ class SeedFeatures:
             self.hist[0, c].copy_(e[c])
         self.nhist += 1
 
-    def set_seed(self, corr, scale):
-        """epw <- kind-1 forward transform of 2 B(j-1) - B(j-2) + scale
-        up(corr): corr is the network's (2, H, W) output."""
-        p, ld, cs = self.tmp.device(0)
-        qpnet.seed_apply(corr, self.H, self.W, self.f, scale, self.hist, p, self.fp64, ld, cs,
-                         self.ny + 1, self.nx + 1)
-        self.tmp.fftrk(1)
-        self.sim.field(qps.F_EPW).asa(self.tmp)
+    def seed(self, corr=None, scale=1.0):
+        """Between prepare and finish: epw <- 2 B(j-1) - B(j-2) + scale up(corr)
+        in k space once two slices are recorded (the correction through the
+        kind-1 forward transform; corr None: no correction), else epw is kept
+        (the previous slice's field); B(j-1) enters the k-space history."""
+        epw = self.sim.field(qps.F_EPW)
+        pe, _, cs = epw.device(1)
+        ph = self.hk.device(1)[0]
+        extrap = self.nk >= 2
+        pc = None
+        if extrap and corr is not None:
+            pt, ld, cst = self.tmp.device(0)
+            qpnet.seed_apply(corr, self.H, self.W, self.f, scale, None, pt, self.fp64, ld, cst,
+                             self.ny + 1, self.nx + 1)
+            self.tmp.fftrk(1)
+            pc = self.tmp.device(1)[0]
+        qpnet.seed_combine(pe, ph, pc, 2 * cs, extrap, self.fp64)
+        self.nk += 1
 
     def reset(self):
         self.nhist = 0
+        self.nk = 0
 
 
 class LearnedSeed:
@@ -137,10 +151,11 @@ class LearnedSeed:
         fs = self.fs
         with torch.cuda.stream(self.stream):
             self.sim.prepare(j)
+            out = None
             if fs.nhist >= 2:
                 x = fs.gather()
                 out = self.net.forward(x, fs.H * fs.W, fs.W, fs.H, fs.W)
-                fs.set_seed(out, self.net.out_scale)
+            fs.seed(out, self.net.out_scale)
             it, res = self.sim.finish(j, iter_max=iter_max, tol=tol)
             fs.record()
         return it, res

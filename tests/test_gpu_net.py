@@ -247,3 +247,21 @@ def test_block_mean_and_seed_apply(f):
         got = np.stack([d[c * pl: c * pl + (ny + 1) * ldd].reshape(ny + 1, ldd)[:, : nx + 1]
                         for c in range(2)])
         assert np.abs(got - want).max() < 2e-5 * np.abs(want).max()
+
+
+def test_seed_combine():
+    banner("seed combine")
+    rng = np.random.default_rng(31)
+    n = 1000
+    for dbl in (False, True):
+        dt = torch.float64 if dbl else torch.float32
+        e0, h0, c0 = (rng.standard_normal(n) for _ in range(3))
+        e = torch.tensor(e0, dtype=dt, device="cuda")
+        h = torch.tensor(h0, dtype=dt, device="cuda")
+        c = torch.tensor(c0, dtype=dt, device="cuda")
+        qpnet.seed_combine(e.data_ptr(), h.data_ptr(), c.data_ptr(), n, True, dbl)
+        tol = 1e-12 if dbl else 1e-5
+        assert np.abs(e.cpu().numpy() - (2 * e0 - h0 + c0)).max() < tol * 10
+        assert np.abs(h.cpu().numpy() - e0).max() < tol
+        qpnet.seed_combine(e.data_ptr(), h.data_ptr(), None, n, False, dbl)   # record only
+        assert np.array_equal(h.cpu().numpy(), e.cpu().numpy())

@@ -1,0 +1,126 @@
+"""The learned corrector seed in the slice loop (quickpic-opensource_amd/qpseed.py).
+
+The network only sets where the corrector starts (QP_TUNE_SEED = 2); the
+plain iteration still runs to the residual tolerance, so:
+  * with a zero correction the caller-provided seed is the extrapolated one,
+    and slices end with the same pass counts and fields (within the
+    tolerance) as the library's own extrapolated seed (QP_TUNE_SEED = 1);
+  * any seed, even a poor one, ends at the tolerance: fields agree with the
+    previous-slice seed's to the level the tolerance allows;
+  * the features fed to the network are the block means of the planes they
+    name (checked against the planes downloaded through the C ABI).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+qps = pytest.importorskip("qpslice")
+qpnet = pytest.importorskip("qpnet")
+qpseed = pytest.importorskip("qpseed")
+
+pytestmark = pytest.mark.gpu
+
+IND, INDZ = 6, 6
+BOX = (-10.0, 10.0, -10.0, 10.0, 0.0, 10.0)
+TOL = 1e-6
+
+
+def banner(name):
+    print(f"[gpu-test] {name}", flush=True)
+
+
+def make(precision=qps.FP32):
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=precision)
+    sp = qps.Species(ctx, ppc=(2, 2))
+    bm = qps.Beam(ctx, center=(0.0, 0.0, 1.5), sigma=(0.4, 0.4, 0.5), np_=(32, 32, 64),
+                  peak=8.0, dt=10.0)
+    sim = qps.Sim(ctx, [sp], [bm], iter_=2)
+    sim.begin_step()
+    return ctx, sim
+
+
+def zero_net():
+    net = qpnet.UNet(seed=1)
+    net.hw.zero_()
+    net.hb.zero_()
+    return net
+
+
+def fields(sim):
+    F = sim.field
+    return [F(qps.F_PSIT).download(0), F(qps.F_BXYZ).download(0), F(qps.F_FXY).download(0)]
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("f", [1, 2])
+def test_zero_correction_is_the_extrapolated_seed(f):
+    banner(f"zero-correction seed f={f}")
+    ctx1, sim1 = make()
+    ctx1.tune(qps.TUNE_SEED, 1)
+    ctx2, sim2 = make()
+    ls = qpseed.LearnedSeed(ctx2, sim2, zero_net(), f)
+    p1, p2 = [], []
+    for j in range(1, 25):
+        p1.append(sim1.slice(j, iter_max=20, tol=TOL)[0])
+        p2.append(ls.slice(j, iter_max=20, tol=TOL)[0])
+    print("passes, library extrapolation:", p1, flush=True)
+    print("passes, caller seed          :", p2, flush=True)
+    assert sum(abs(a - b) for a, b in zip(p1, p2)) <= 2
+    for a, b in zip(fields(sim2), fields(sim1)):
+        assert rel(a, b) < 1e-4
+
+
+def test_any_seed_converges_to_the_same_fields():
+    banner("random-network seed")
+    ctx0, sim0 = make()
+    ctx0.tune(qps.TUNE_SEED, 0)
+    ctx2, sim2 = make()
+    net = qpnet.UNet(seed=7)
+    net.out_scale = 1e-3
+    ls = qpseed.LearnedSeed(ctx2, sim2, net, 2)
+    for j in range(1, 21):
+        it0, r0 = sim0.slice(j, iter_max=30, tol=TOL)
+        it2, r2 = ls.slice(j, iter_max=30, tol=TOL)
+        assert it0 < 30 and it2 < 30 and r0 < TOL and r2 < TOL
+    for a, b in zip(fields(sim2), fields(sim0)):
+        assert rel(a, b) < 1e-4
+
+
+def test_features_are_block_means_of_the_named_planes():
+    banner("seed features")
+    ctx, sim = make()
+    ctx.tune(qps.TUNE_SEED, 2)
+    with torch.cuda.stream(qpseed.ctx_stream(ctx)):
+        fs = qpseed.SeedFeatures(ctx, sim, 2)
+        for j in range(1, 6):
+            sim.prepare(j)
+            x = fs.gather()
+            ctx.sync()
+            if j == 5:
+                got = x.cpu().numpy().astype(np.float64)
+            sim.finish(j, iter_max=20, tol=TOL)
+            if j == 3:
+                b3 = sim.field(qps.F_EPW).download(1)
+            fs.record()
+        ctx.sync()
+    ny, nx = ctx.ny, ctx.nx
+    H, W = ny // 2, nx // 2
+
+    def bm(p):
+        return p[:ny, :nx].reshape(H, 2, W, 2).mean(axis=(1, 3))
+
+    F = sim.field
+    # slice 5's plasma charge: only prepare writes qe, so it is still there
+    qe = F(qps.F_QE).download(0)[0]
+    assert rel(got[4], bm(qe)) < 1e-5
+    # B_perp of slice 3 (history entry 1 at slice 5): its real-space form by
+    # the library's own inverse transform of the downloaded k-space field
+    tmp = qps.Field2d(ctx, 2, state=1)
+    tmp.upload(b3, space=1)
+    tmp.fftkr(1)
+    b3r = tmp.download(0)
+    for c in range(2):
+        assert rel(got[2 + c], bm(b3r[c])) < 1e-5
