@@ -35,7 +35,10 @@ int DevBuf::alloc(size_t n)
         return QP_ERR_ALLOC;
     }
     bytes = n;
+    // zeroed on the null stream and waited for: the contexts' streams are
+    // non-blocking, so later work on them is not ordered after the null stream
     e = hipMemset(p, 0, n);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return hip_fail(e, "hipMemset", __FILE__, __LINE__);
     return QP_OK;
 }

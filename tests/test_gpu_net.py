@@ -95,9 +95,9 @@ def test_pool_up_head():
     hw = rng.standard_normal((2, C)).astype(np.float32)
     hb = rng.standard_normal(2).astype(np.float32)
     o = torch.zeros(2 * H * W, device="cuda")
-    qpnet._call(L.qp_net_head, qpnet.stream(), qpnet.ptr(xd), C, C,
-                qpnet.ptr(torch.from_numpy(hw).cuda()), qpnet.ptr(torch.from_numpy(hb).cuda()), 2,
-                H * W, qpnet.ptr(o))
+    hwd, hbd = torch.from_numpy(hw).cuda(), torch.from_numpy(hb).cuda()   # held past the launch
+    qpnet._call(L.qp_net_head, qpnet.stream(), qpnet.ptr(xd), C, C, qpnet.ptr(hwd),
+                qpnet.ptr(hbd), 2, H * W, qpnet.ptr(o))
     assert rel(o.cpu().numpy().reshape(2, -1), nr.head(x, hw, hb)) < 1e-5
 
 
