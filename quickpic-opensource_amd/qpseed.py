@@ -1,10 +1,14 @@
 """Learned corrector seed for the slice loop, and the training of its network.
 
 Between qp_slice_prepare and qp_slice_finish of slice j the network
-(qpnet.UNet) reads, in real space on the nx x ny nodes:
-  B_perp of slices j-1 and j-2 (4 channels), the plasma charge of slice j,
-  the current J_x, J_y of the slice before and psi of slice j,
-and predicts the correction of B_perp over 2 B(j-1) - B(j-2).  The seed is
+(qpnet.UNet) reads, in real space on its (coarsened) grid:
+  B_perp of slice j-1 and its change B(j-1) - B(j-2) (4 channels), the
+  plasma charge of slice j, the current J_x, J_y of the slice before and psi
+  of slice j,
+each group divided by its own root mean square over the slice, and predicts
+the correction of B_perp over 2 B(j-1) - B(j-2) in units of the root mean
+square of B(j-1) - B(j-2).  So the network sees fields of order one whatever
+the driver's strength, and its output scales back with the slice.  The seed is
 that extrapolation, formed in k space as the library's own extrapolated seed
 is, plus the correction's kind-1 transform (QP_TUNE_SEED = 2); the plain
 iteration to the residual tolerance is unchanged, so the network only changes
@@ -40,6 +44,11 @@ def ctx_stream(ctx: qps.Ctx):
     return torch.cuda.ExternalStream(ctx.stream())
 
 
+# the normalisation group of each input channel: B(j-1), B(j-1) - B(j-2),
+# charge, current, psi
+GROUP = [0, 0, 1, 1, 2, 3, 3, 4]
+
+
 class SeedFeatures:
     """Features, targets and seeds of one simulation, on the device, on a
     grid coarsened f times (the network's grid: (ny / f) x (nx / f), both
@@ -61,6 +70,8 @@ class SeedFeatures:
                                 device="cuda")
         self.nhist = 0
         self.feat = torch.zeros((qpnet.NIN, self.H, self.W), dtype=torch.float32, device="cuda")
+        self.xn = torch.zeros_like(self.feat)
+        self.sd = self.sb = None
         self.targ = torch.zeros((2, self.H, self.W), dtype=torch.float32, device="cuda")
         self.diff = torch.zeros((2, self.ny + 1, self.nx + 1), dtype=torch.float32,
                                 device="cuda")
@@ -82,9 +93,9 @@ class SeedFeatures:
         return (t.data_ptr(), self.nx + 1, False)
 
     def gather(self):
-        """After qp_slice_prepare: the 8 input planes on the network's grid:
-        B_perp of the last two slices, this slice's plasma charge, the
-        previous slice's J_x, J_y and this slice's psi."""
+        """After qp_slice_prepare: the 8 normalised input planes on the
+        network's grid (self.xn); self.sd is the output unit (a device scalar,
+        the root mean square of B(j-1) - B(j-2))."""
         h = self.hist
         cu = self.sim.field(qps.F_CU)
         planes = [self._hist_plane(h[0, 0]), self._hist_plane(h[0, 1]),
@@ -92,13 +103,22 @@ class SeedFeatures:
                   self._real_plane(self.sim.field(qps.F_QE), 0),
                   self._real_plane(cu, 0), self._real_plane(cu, 1),
                   self._real_plane(self.sim.field(qps.F_PSI), 0)]
-        qpnet.block_mean(planes, self.f, self.H, self.W, self.feat)
-        return self.feat
+        f = self.feat
+        qpnet.block_mean(planes, self.f, self.H, self.W, f)
+        f[2:4] = f[0:2] - f[2:4]
+        ms = (f * f).mean(dim=(1, 2))
+        g = torch.stack([ms[0:2].mean(), ms[2:4].mean(), ms[4], ms[5:7].mean(), ms[7]])
+        g = g.sqrt().clamp(min=1e-30)
+        torch.div(f, g[GROUP][:, None, None], out=self.xn)
+        self.sd = g[1]
+        self.sb = g[0]
+        return self.xn
 
     def record(self, target=False):
         """After qp_slice_finish: the converged B_perp (real space) enters
-        the history; with target, first the network's target, the block
-        means of converged minus extrapolated B_perp, into self.targ."""
+        the history; with target (gather was called for this slice), first
+        the network's target, the block means of converged minus
+        extrapolated B_perp in units of self.sd, into self.targ."""
         self.tmp.asa(self.sim.field(qps.F_EPW))
         self.tmp.fftkr(1)
         e = [self.view(self.tmp, c) for c in range(2)]
@@ -108,6 +128,7 @@ class SeedFeatures:
                           out=self.diff[c])
             qpnet.block_mean([self._hist_plane(self.diff[0]), self._hist_plane(self.diff[1])],
                              self.f, self.H, self.W, self.targ)
+            self.targ /= self.sd
         self.hist[1].copy_(self.hist[0])
         for c in range(2):
             self.hist[0, c].copy_(e[c])
@@ -143,6 +164,7 @@ class LearnedSeed:
     def __init__(self, ctx: qps.Ctx, sim: qps.Sim, net: qpnet.UNet, f=1):
         self.ctx, self.sim, self.net = ctx, sim, net
         ctx.tune(qps.TUNE_SEED, 2)
+        net.in_scale.fill_(1.0)   # the features arrive normalised
         self.stream = ctx_stream(ctx)
         with torch.cuda.stream(self.stream):
             self.fs = SeedFeatures(ctx, sim, f)
@@ -155,7 +177,8 @@ class LearnedSeed:
             if fs.nhist >= 2:
                 x = fs.gather()
                 out = self.net.forward(x, fs.H * fs.W, fs.W, fs.H, fs.W)
-            fs.seed(out, self.net.out_scale)
+                out *= fs.sd
+            fs.seed(out)
             it, res = self.sim.finish(j, iter_max=iter_max, tol=tol)
             fs.record()
         return it, res
@@ -182,33 +205,28 @@ def collect(ctx, sim, j0, j1, f=1, iter_max=20, tol=1e-6, every=1, sink=None):
             it, _ = sim.finish(j, iter_max=iter_max, tol=tol)
             passes.append(it)
             fs.record(target=take)
-            if take:
-                sink(fs.feat, fs.targ)
+            # slices where B has not started to change carry no signal
+            if take and float(fs.sd) > 1e-6 * float(fs.sb):
+                sink(fs.xn, fs.targ)
     ctx.tune(qps.TUNE_SEED, -1)
     return passes
 
 
 class Trainer:
-    """Online training of the network on (features, target) pairs as the
-    plain solver produces them.  The first `warm` pairs fix the per-channel
-    input scales and the output scale (root mean squares) and fill a replay
-    buffer; every later pair replaces a random buffer entry and is followed
-    by `per_pair` Adam steps on random entries."""
+    """Online training of the network on normalised (features, target)
+    pairs as the plain solver produces them: every pair enters a replay
+    buffer (replacing a random entry once it is full) and is followed by
+    `per_pair` Adam steps on random entries, after the first `warm` pairs."""
 
     def __init__(self, net: qpnet.UNet, H, W, buf=64, warm=16, per_pair=1, lr=1e-3, seed=0):
         self.net, self.H, self.W = net, H, W
+        net.in_scale.fill_(1.0)
+        net.out_scale = 1.0
         self.x = torch.zeros((buf, qpnet.NIN, H, W), device="cuda")
         self.y = torch.zeros((buf, 2, H, W), device="cuda")
         self.n, self.warm, self.per_pair, self.lr = 0, warm, per_pair, lr
         self.rng = np.random.default_rng(seed)
         self.losses = []
-
-    def _fit_scales(self):
-        k = min(self.n, len(self.x))
-        xs = self.x[:k].transpose(0, 1).reshape(qpnet.NIN, -1)
-        rms = torch.sqrt((xs * xs).mean(dim=1))
-        self.net.in_scale.copy_(1.0 / torch.clamp(rms, min=1e-20))
-        self.net.out_scale = float(torch.sqrt((self.y[:k] ** 2).mean()).clamp(min=1e-20))
 
     def add(self, x, y):
         b = len(self.x)
@@ -216,8 +234,6 @@ class Trainer:
         self.x[i].copy_(x)
         self.y[i].copy_(y)
         self.n += 1
-        if self.n == self.warm:
-            self._fit_scales()
         if self.n >= self.warm:
             for _ in range(self.per_pair):
                 self.step()
@@ -227,8 +243,7 @@ class Trainer:
         i = int(self.rng.integers(k))
         H, W = self.H, self.W
         self.net.forward(self.x[i], H * W, W, H, W)
-        t = (self.y[i] / self.net.out_scale).reshape(2, -1)
-        self.losses.append(self.net.backward(t))
+        self.losses.append(self.net.backward(self.y[i].reshape(2, -1)))
         self.net.adam(lr=self.lr)
 
 

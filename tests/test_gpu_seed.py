@@ -78,8 +78,7 @@ def test_any_seed_converges_to_the_same_fields():
     ctx0, sim0 = make()
     ctx0.tune(qps.TUNE_SEED, 0)
     ctx2, sim2 = make()
-    net = qpnet.UNet(seed=7)
-    net.out_scale = 1e-3
+    net = qpnet.UNet(seed=7)   # untrained: a correction of the size of B's change
     ls = qpseed.LearnedSeed(ctx2, sim2, net, 2)
     for j in range(1, 21):
         it0, r0 = sim0.slice(j, iter_max=30, tol=TOL)
@@ -93,17 +92,20 @@ def test_features_are_block_means_of_the_named_planes():
     banner("seed features")
     ctx, sim = make()
     ctx.tune(qps.TUNE_SEED, 2)
+    bk = {}
     with torch.cuda.stream(qpseed.ctx_stream(ctx)):
         fs = qpseed.SeedFeatures(ctx, sim, 2)
         for j in range(1, 6):
             sim.prepare(j)
-            x = fs.gather()
+            fs.gather()
             ctx.sync()
             if j == 5:
-                got = x.cpu().numpy().astype(np.float64)
+                raw = fs.feat.cpu().numpy().astype(np.float64)
+                xn = fs.xn.cpu().numpy().astype(np.float64)
+                sd = float(fs.sd)
+            fs.seed()
             sim.finish(j, iter_max=20, tol=TOL)
-            if j == 3:
-                b3 = sim.field(qps.F_EPW).download(1)
+            bk[j] = sim.field(qps.F_EPW).download(1)
             fs.record()
         ctx.sync()
     ny, nx = ctx.ny, ctx.nx
@@ -112,15 +114,27 @@ def test_features_are_block_means_of_the_named_planes():
     def bm(p):
         return p[:ny, :nx].reshape(H, 2, W, 2).mean(axis=(1, 3))
 
+    def real(k):
+        """B_perp in real space by the library's own inverse transform."""
+        tmp = qps.Field2d(ctx, 2, state=1)
+        tmp.upload(k, space=1)
+        tmp.fftkr(1)
+        return tmp.download(0)
+
     F = sim.field
     # slice 5's plasma charge: only prepare writes qe, so it is still there
     qe = F(qps.F_QE).download(0)[0]
-    assert rel(got[4], bm(qe)) < 1e-5
-    # B_perp of slice 3 (history entry 1 at slice 5): its real-space form by
-    # the library's own inverse transform of the downloaded k-space field
-    tmp = qps.Field2d(ctx, 2, state=1)
-    tmp.upload(b3, space=1)
-    tmp.fftkr(1)
-    b3r = tmp.download(0)
+    assert rel(raw[4], bm(qe)) < 1e-5
+    # B(4) and B(4) - B(3), the history at slice 5
+    b4, b3 = real(bk[4]), real(bk[3])
     for c in range(2):
-        assert rel(got[2 + c], bm(b3r[c])) < 1e-5
+        assert rel(raw[c], bm(b4[c])) < 1e-5
+        assert rel(raw[2 + c], bm(b4[c] - b3[c])) < 1e-4
+    # each group over its own root mean square; sd is the change's
+    grp = qpseed.GROUP
+    for g in range(5):
+        ch = [c for c in range(8) if grp[c] == g]
+        r = np.sqrt(np.mean(raw[ch] ** 2))
+        assert rel(xn[ch], raw[ch] / r) < 1e-5
+        if g == 1:
+            assert abs(sd / r - 1) < 1e-5
