@@ -93,6 +93,15 @@ int qp_net_adam(void *stream, float *wm, const float *g, float *m, float *v, siz
                 float b1, float b2, float eps, int step, uint16_t *wb);
 int qp_net_to_bf16(void *stream, const float *a, size_t n, uint16_t *b);
 
+/* HIP-graph capture of a fixed launch sequence (the network's forward pass
+ * on fixed buffers): begin capturing the stream's work, end it into an
+ * executable graph, replay it on a stream, destroy it.  The captured
+ * sequence contains kernel launches only (no memset or copy nodes). */
+int qp_net_graph_begin(void *stream);
+int qp_net_graph_end(void *stream, void **graph_exec);
+int qp_net_graph_launch(void *graph_exec, void *stream);
+int qp_net_graph_destroy(void *graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

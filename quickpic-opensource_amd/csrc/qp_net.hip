@@ -627,6 +627,38 @@ int qp_net_seed_combine(void *stream_, void *epw, void *hist, const void *corr, 
     return QP_OK;
 }
 
+int qp_net_graph_begin(void *stream_)
+{
+    QP_HIP(hipStreamBeginCapture((hipStream_t)stream_, hipStreamCaptureModeThreadLocal));
+    return QP_OK;
+}
+
+int qp_net_graph_end(void *stream_, void **graph_exec)
+{
+    NET_CHECK(graph_exec, "qp_net_graph_end: null output");
+    hipGraph_t g = nullptr;
+    QP_HIP(hipStreamEndCapture((hipStream_t)stream_, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    QP_HIP(e);
+    *graph_exec = (void *)ge;
+    return QP_OK;
+}
+
+int qp_net_graph_launch(void *graph_exec, void *stream_)
+{
+    NET_CHECK(graph_exec, "qp_net_graph_launch: null graph");
+    QP_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream_));
+    return QP_OK;
+}
+
+int qp_net_graph_destroy(void *graph_exec)
+{
+    if (graph_exec) QP_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return QP_OK;
+}
+
 int qp_net_head(void *stream_, const uint16_t *h, int cs, int cin, const float *w,
                 const float *b, int cout, size_t npix, float *o)
 {

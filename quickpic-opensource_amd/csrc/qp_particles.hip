@@ -450,6 +450,120 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         emit_nodes(nacc + c * 289, out, c, g.ldr, halo + c * halo_cs, tile, tx, ty);
 }
 
+// The same deposit with the next chunk's particles loaded into registers
+// while the current chunk is computed, so a workgroup's global loads overlap
+// its own arithmetic (the staging chunk of k_amjdp waits for its loads).
+// The tile's particles are one contiguous run, row group after row group, so
+// the chunk after the last of a group is the first of the next; a chunk that
+// was not prefetched (after an empty group) is loaded when it is reached.
+// Cells, lanes, sums and their order are those of k_amjdp with the same RG.
+template <typename R, int RG, int CH>
+__global__ void __launch_bounds__(256)
+k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
+           const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
+           const int *skip)
+{
+    if (skip && *skip) return;
+    constexpr int T = TILE / RG, NP = Stage<CH>::N, PF = CH / TCELLS;
+    extern __shared__ unsigned char smem_raw[];
+    R *sx = reinterpret_cast<R *>(smem_raw), *sy = sx + NP, *spx = sy + NP, *spy = spx + NP;
+    R *sp6 = spy + NP, *sp7 = sp6 + NP, *sq = sp7 + NP, *nacc = sq + NP;   // nacc: 8 x 289
+    const int tid = threadIdx.x, tile = blockIdx.x;
+    const int tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
+    for (int n = tid; n < 8 * 289; n += TCELLS) nacc[n] = R(0);
+    const int cl = tid / T, sub = tid % T;
+    const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
+    R rx[PF], ry[PF], rpx[PF], rpy[PF], rp6[PF], rp7[PF], rq[PF];
+    int pc0 = -1, pn = 0;   // the chunk held in the registers
+    auto fetch = [&](int b0, int nn) {
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int p = tid + k * TCELLS;
+            if (p < nn) {
+                const int q = b0 + p;
+                rx[k] = s.x[q];
+                ry[k] = s.y[q];
+                rpx[k] = s.px[q];
+                rpy[k] = s.py[q];
+                rp6[k] = s.p6[q];
+                rp7[k] = s.p7[q];
+                rq[k] = s.q[q];
+            }
+        }
+        pc0 = b0;
+        pn = nn;
+    };
+    for (int row = 0; row < TILE; row += RG) {
+        const int lrow = row + cl / TILE, lcol = cl % TILE;
+        const int key = kb + lrow * TILE + lcol;
+        const int i0 = off[key], i1 = off[key + 1];
+        const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
+        const int G2 = row + RG < TILE ? off[kb + (row + 2 * RG) * TILE] : G1;
+        const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
+        const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        const R fx = (R)cx, fy = (R)cy;
+        R acc[8][4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
+        for (int c0 = G0; c0 < G1; c0 += CH) {
+            const int n = min(CH, G1 - c0);
+            if (pc0 != c0 || pn != n) fetch(c0, n);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const int p = tid + k * TCELLS;
+                if (p < n) {
+                    const int l = lpad(p);
+                    sx[l] = rx[k];
+                    sy[l] = ry[k];
+                    spx[l] = rpx[k];
+                    spy[l] = rpy[k];
+                    sp6[l] = rp6[k];
+                    sp7[l] = rp7[k];
+                    sq[l] = rq[k];
+                }
+            }
+            __syncthreads();
+            // the next chunk: the rest of this group, or the start of the next
+            const int nc0 = c0 + n;
+            const int nn = nc0 < G1 ? min(CH, G1 - nc0) : min(CH, G2 - nc0);
+            if (nn > 0) fetch(nc0, nn);
+            else pc0 = -1;
+            int a = i0 + sub;
+            if (a < c0) a += ((c0 - a + T - 1) / T) * T;
+            const int b = min(i1, c0 + n);
+            for (int i = a; i < b; i += T) {
+                const int l = lpad(i - c0);
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], sp6[l], sp7[l],
+                                   sq[l], qtmh, qbm, dti, dex, idex, acc);
+            }
+            __syncthreads();
+            for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
+        }
+#pragma unroll
+        for (int o = T / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[c][k] += __shfl_xor(acc[c][k], o);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            __syncthreads();
+            if (sub == 0) {
+                const int node = (lrow + (k >> 1)) * 17 + lcol + (k & 1);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) nacc[c * 289 + node] += acc[c][k];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        emit_nodes(nacc + c * 289, out, c, g.ldr, halo + c * halo_cs, tile, tx, ty);
+}
+
 // Orders LDS accesses among the lanes of one wave (no workgroup barrier).
 __device__ __forceinline__ void wave_sync()
 {
@@ -888,7 +1002,8 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
 // Variant 0 of each table is the default; qp_ctx_tune selects another.
 struct DepShape {
     int rg, ch, wpe;
-    int wave;          // 1: k_amjdp_w (independent waves, staging chunk ch per wave)
+    int wave;          // 1: k_amjdp_w (independent waves, staging chunk ch per wave);
+                       // 2: k_amjdp_pf (next chunk prefetched into registers)
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
@@ -896,14 +1011,16 @@ template <> struct DepTab<float> {
                                      {16, 2048, 1, 0}};
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
-                                     {2, 256, 1, 1},   {2, 128, 1, 1}};
+                                     {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
+                                     {8, 512, 1, 2}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
                                      {8, 512, 1, 0}};
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
-                                     {2, 128, 1, 1},  {2, 64, 1, 1}};
+                                     {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
+                                     {8, 256, 1, 2}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1051,9 +1168,12 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     size_t lds;
     void (*kern)(TileGeom, Soa<R>, const int *, const R *, const R *, const R *, size_t,
                  DepOut<R>, R *, size_t, R, R, R, const int *);
-    if constexpr (d.wave) {
+    if constexpr (d.wave == 1) {
         lds = (4 * 7 * (size_t)Stage<d.ch>::N + 8 * 8 * 51) * sizeof(R);
         kern = k_amjdp_w<R, d.ch>;
+    } else if constexpr (d.wave == 2) {
+        lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
+        kern = k_amjdp_pf<R, d.rg, d.ch>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
@@ -1084,6 +1204,8 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 5: rc = amjdp_v<R, 5>(sp, ef, bf, psit, dex, out); break;
     case 6: rc = amjdp_v<R, 6>(sp, ef, bf, psit, dex, out); break;
     case 7: rc = amjdp_v<R, 7>(sp, ef, bf, psit, dex, out); break;
+    case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out); break;
+    case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out); break;
     }
     if (rc) return rc;

@@ -49,6 +49,10 @@ def _lib():
                                   fp],
             "qp_net_seed_apply": [vp, fp, i, i, i, f, fp, fp, i, sz, vp, i, i, sz, i, i],
             "qp_net_seed_combine": [vp, vp, vp, vp, sz, i, i],
+            "qp_net_graph_begin": [vp],
+            "qp_net_graph_end": [vp, C.POINTER(C.c_void_p)],
+            "qp_net_graph_launch": [vp, vp],
+            "qp_net_graph_destroy": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -175,6 +179,7 @@ class UNet:
         self.out_scale = 1.0
         self.step = 0
         self._shape = None
+        self._graph = None     # (key, executable graph) of the captured forward
 
     # -- buffers ------------------------------------------------------------
     def _alloc(self, H, W):
@@ -250,6 +255,31 @@ class UNet:
         _call(_lib().qp_net_head, stream(), ptr(a["d1b"]), c1, c1, ptr(self.hw), ptr(self.hb), NOUT,
               H * W, ptr(self.out))
         return self.out
+
+    def forward_graph(self, planes, plane, ld, H, W):
+        """forward() replayed from a HIP graph captured on the first call
+        with these buffers and sizes (the weights are read at replay, so
+        training between replays is seen); returns the output buffer."""
+        key = (planes.data_ptr(), plane, ld, H, W, torch.cuda.current_stream().cuda_stream)
+        if self._graph is None or self._graph[0] != key:
+            self.drop_graph()
+            self.forward(planes, plane, ld, H, W)   # buffers allocated outside the capture
+            st = stream()
+            _call(_lib().qp_net_graph_begin, st)
+            try:
+                self.forward(planes, plane, ld, H, W)
+            finally:
+                ge = C.c_void_p()
+                _call(_lib().qp_net_graph_end, st, C.byref(ge))
+            self._graph = (key, ge)
+            return self.out
+        _call(_lib().qp_net_graph_launch, self._graph[1], stream())
+        return self.out
+
+    def drop_graph(self):
+        if self._graph is not None:
+            _call(_lib().qp_net_graph_destroy, self._graph[1])
+            self._graph = None
 
     # -- backward -------------------------------------------------------------
     def _conv_bwd(self, cv, x, cs_in, H, W, y, gy, cs_out, gx, x_off=0, y_off=0, gx_off=0,

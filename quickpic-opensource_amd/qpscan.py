@@ -1,0 +1,173 @@
+"""Batched parameter scan: many independent slice-solver instances on one GPU.
+
+Each instance is a context with its own HIP stream, plasma species and
+bi-Gaussian driver; instances differ in transverse grid, particles per cell
+and driver parameters (nb/n0 ~ U[1, 20], sigma_r ~ U[0.2, 1.0], sigma_z ~
+U[0.3, 1.5], per-instance beam seed).  The transverse grids are powers of
+two, as the reference's are (indx, indy are log2 sizes,
+simulation_class.f03:151-224), so a scan draws them from 2^indx choices.
+
+run_batched enqueues slice j of every instance before slice j+1 of any, each
+on its instance's stream, so the GPU works on several instances at once; the
+corrector runs the reference's fixed number of passes (no host wait inside a
+slice).  run_sequential runs the instances one after the other.  The two give
+bitwise-equal fields per instance (every kernel is deterministic and an
+instance's work stays on its own stream), which tests/test_gpu_scan.py
+checks.
+
+autotune() picks, per instance, the fastest launch shape of each deposit
+kernel (qp_ctx_tune) by timing a few slices of that instance with the
+library's event profiler.
+
+usage: python quickpic-opensource_amd/qpscan.py --n 64 --grids 8 9 --slices 256
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+if _HERE not in sys.path:
+    sys.path.insert(0, _HERE)
+
+import qpslice as qps  # noqa: E402
+
+BOX = (-10.0, 10.0, -10.0, 10.0, 0.0, 10.0)
+
+
+class Instance:
+    def __init__(self, ind, indz, ppc, peak, sr, sz, seed, beam_np, precision=qps.FP32,
+                 iter_=2):
+        self.params = {"grid": 2 ** ind, "ppc": ppc * ppc, "peak": peak, "sigma_r": sr,
+                       "sigma_z": sz, "seed": seed}
+        self.ctx = qps.Ctx(ind, ind, indz, box=BOX, precision=precision)
+        self.sp = qps.Species(self.ctx, ppc=(ppc, ppc))
+        self.bm = qps.Beam(self.ctx, center=(0.0, 0.0, 2.0), sigma=(sr, sr, sz),
+                           sigma_v=(1.0, 1.0, 0.0), np_=beam_np, gamma=2.0e4, peak=peak,
+                           dt=10.0, seed=seed)
+        self.sim = qps.Sim(self.ctx, [self.sp], [self.bm], iter_=iter_)
+        self.iter = iter_
+
+    def begin(self):
+        self.sp.renew(0.0)
+        self.sim.begin_step()
+
+    def slice(self, j):
+        self.sim.slice(j, iter_max=self.iter, tol=0.0)
+
+    def fields(self):
+        F = self.sim.field
+        return [F(qps.F_PSIT).download(0), F(qps.F_BXYZ).download(0),
+                F(qps.F_FXY).download(0), F(qps.F_CU).download(1)]
+
+    def close(self):
+        self.ctx.close()
+
+
+def draw(n, grids, ppcs, indz, beam_np, seed=0, precision=qps.FP32):
+    """n instances with grids 2^g (g from grids), ppc^2 particles per cell
+    (ppc from ppcs) and drivers drawn from the ranges above."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        g = int(rng.choice(grids))
+        p = int(rng.choice(ppcs))
+        peak, sr, sz = (float(rng.uniform(1, 20)), float(rng.uniform(0.2, 1.0)),
+                        float(rng.uniform(0.3, 1.5)))
+        out.append(Instance(g, indz, p, peak, sr, sz, seed=1 + 7919 * seed + i,
+                            beam_np=beam_np, precision=precision))
+    return out
+
+
+def autotune(inst, j0, reps=2):
+    """Fastest variant of each deposit kernel for this instance, timed over
+    `reps` slices per variant starting at slice j0 (the slices are computed
+    and the instance must be restarted afterwards); returns {knob: value}."""
+    ctx, best = inst.ctx, {}
+    for knob, kind in ((qps.TUNE_AMJDP, "amjdp"), (qps.TUNE_DEPOSIT_Q, "deposit_q")):
+        _, nv = ctx.tune_get(knob)
+        k = qps.PROF_NAMES.index(kind)
+        times = []
+        j = j0
+        for v in range(nv):
+            ctx.tune(knob, v)
+            ctx.profile(True)
+            for _ in range(reps):
+                inst.slice(j)
+                j += 1
+            t, cnt, _ = ctx.profile_get(k)
+            ctx.profile(False)
+            times.append(t / max(cnt, 1))
+        best[knob] = int(np.argmin(times))
+        ctx.tune(knob, best[knob])
+    return best
+
+
+def run_batched(insts, nslices):
+    for x in insts:
+        x.begin()
+    for x in insts:
+        x.ctx.sync()
+    t = time.perf_counter()
+    for j in range(1, nslices + 1):
+        for x in insts:
+            x.slice(j)
+    for x in insts:
+        x.ctx.sync()
+    return time.perf_counter() - t
+
+
+def run_sequential(insts, nslices):
+    for x in insts:
+        x.begin()
+    for x in insts:
+        x.ctx.sync()
+    t = time.perf_counter()
+    for x in insts:
+        for j in range(1, nslices + 1):
+            x.slice(j)
+        x.ctx.sync()
+    return time.perf_counter() - t
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--grids", type=int, nargs="+", default=[8, 9],
+                    help="log2 transverse sizes to draw from")
+    ap.add_argument("--ppcs", type=int, nargs="+", default=[2, 3, 4],
+                    help="particles per cell per dimension to draw from")
+    ap.add_argument("--indz", type=int, default=8)
+    ap.add_argument("--slices", type=int, default=256)
+    ap.add_argument("--beam-np", type=int, nargs=3, default=[32, 32, 256])
+    ap.add_argument("--tune", action="store_true", help="autotune each instance first")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.time()
+    insts = draw(args.n, args.grids, args.ppcs, args.indz, tuple(args.beam_np), args.seed)
+    log(f"[scan] {args.n} instances set up in {time.time() - t0:.1f}s")
+    if args.tune:
+        for x in insts:
+            x.begin()
+            x.params["tune"] = autotune(x, 1)
+    ts = run_sequential(insts, args.slices)
+    tb = run_batched(insts, args.slices)
+    total = args.n * args.slices
+    res = {"instances": args.n, "slices_each": args.slices, "sequential_slices_per_s": total / ts,
+           "batched_slices_per_s": total / tb, "speedup": ts / tb,
+           "grids": sorted({x.params["grid"] for x in insts})}
+    log(f"[scan] sequential {total / ts:.1f} slices/s, batched {total / tb:.1f} slices/s "
+        f"({ts / tb:.2f}x)")
+    for x in insts:
+        x.close()
+    import json
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

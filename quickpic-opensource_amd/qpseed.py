@@ -176,7 +176,7 @@ class LearnedSeed:
             out = None
             if fs.nhist >= 2:
                 x = fs.gather()
-                out = self.net.forward(x, fs.H * fs.W, fs.W, fs.H, fs.W)
+                out = self.net.forward_graph(x, fs.H * fs.W, fs.W, fs.H, fs.W)
                 out *= fs.sd
             fs.seed(out)
             it, res = self.sim.finish(j, iter_max=iter_max, tol=tol)

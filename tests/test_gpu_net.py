@@ -265,3 +265,31 @@ def test_seed_combine():
         assert np.abs(h.cpu().numpy() - e0).max() < tol
         qpnet.seed_combine(e.data_ptr(), h.data_ptr(), None, n, False, dbl)   # record only
         assert np.array_equal(h.cpu().numpy(), e.cpu().numpy())
+
+
+def test_forward_graph_replays_match_eager():
+    """The captured forward replays to the eager result, with eager work and
+    synchronisations between replays, and sees weight updates (Adam writes
+    the BF16 weights in place)."""
+    banner("forward graph")
+    H = W = 32
+    rng = np.random.default_rng(12)
+    net = qpnet.UNet(seed=5)
+    x = torch.from_numpy(rng.standard_normal((qpnet.NIN, H, W)).astype(np.float32)).cuda()
+    want = net.forward(x, H * W, W, H, W).clone()
+    for r in range(4):
+        got = net.forward_graph(x, H * W, W, H, W).clone()
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), r
+        y = torch.randn(1000, device="cuda").sum()   # eager work between replays
+        torch.cuda.synchronize()
+        del y
+    tgt = torch.zeros(2, H * W, device="cuda")
+    net.backward(tgt)
+    net.adam(lr=1e-2)
+    want2 = net.forward(x, H * W, W, H, W).clone()
+    assert not torch.equal(want2, want)
+    got2 = net.forward_graph(x, H * W, W, H, W).clone()
+    torch.cuda.synchronize()
+    assert torch.equal(got2, want2)
+    net.drop_graph()

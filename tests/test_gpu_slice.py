@@ -91,6 +91,30 @@ def test_slices_fp32_vs_fp64_oracle():
     ctx.close()
 
 
+@pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
+def test_every_fp32_deposit_shape_matches_oracle(knob):
+    """The FP32 launch shapes (their own table) against the FP64 oracle at
+    the FP32 bar, 1e-4 relative after 4 slices."""
+    banner(f"fp32 deposit shapes {knob}")
+    ctx0 = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP32)
+    _, nv = ctx0.tune_get(getattr(qps, knob))
+    ctx0.close()
+    osim = None
+    for v in range(nv):
+        ctx, o, gsim, sp = make_pair(qps.FP32)
+        if osim is None:   # one oracle run serves every shape
+            osim = o
+            osim.begin_step()
+            for j in range(1, 5):
+                osim.slice(j)
+        ctx.tune(getattr(qps, knob), v)
+        gsim.begin_step()
+        for j in range(1, 5):
+            gsim.slice(j)
+        compare_state(osim, gsim, sp, 4, 1e-4)
+        ctx.close()
+
+
 def test_adaptive_iteration_reaches_tolerance():
     banner("adaptive corrector")
     ctx, osim, gsim, sp = make_pair(qps.FP64, peak=10.0, sigma=(0.3, 0.3, 0.5))
