@@ -299,12 +299,13 @@ def main():
         # relative error of the precision form against the FP64 path on the
         # same beam: max |a32 - a64| / max |a64| over the sampled wake slices
         common = sorted(set(r32["wake"]) & set(r64["wake"]))
-        errs = {}
+        errs, peaks = {}, {}
         for k in ("Ez", "Ex-By", "Bz", "psi"):
             if common:
                 a = np.stack([r32["wake"][j][k] for j in common])
                 b = np.stack([r64["wake"][j][k] for j in common])
                 errs[k] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+                peaks[k] = float(np.abs(b).max())
         if r32["beam_energy"] and r64["beam_energy"]:
             (a0, a1), (b0, b1) = r32["beam_energy"], r64["beam_energy"]
             errs["beam energy"] = abs(a1 - b1) / abs(b1)
@@ -348,6 +349,7 @@ def main():
                              "fp32_over_fp64": round(r32["slices_per_s"] / r64["slices_per_s"], 3),
                              "fp32_rel_err": {k: float(f"{v:.3g}") for k, v in errs.items()},
                              "fp32_rel_err_slices": common,
+                             "fp64_peak": {k: float(f"{v:.4g}") for k, v in peaks.items()},
                              "step_stop": {"fp32": r32["step_stop"], "fp64": r64["step_stop"]},
                              "roofline": r64["roofline"]}
     print(json.dumps(out), flush=True)

@@ -291,6 +291,10 @@ int qp_slice_run(qp_sim *sim, int j, int iter_max, double tol,
 int qp_slice_prepare(qp_sim *sim, int j);
 int qp_slice_finish(qp_sim *sim, int j, int iter_max, double tol, int *iters_used,
                     double *residual);
+/* The corrector residuals of the last slice finished with tol > 0, pass 1
+ * first (at most 63 and at most max of them; *n gets the count).  A
+ * diagnostic with no reference counterpart. */
+int qp_slice_residuals(const qp_sim *sim, double *res, int max, int *n);
 /* end of the 3-D step: beam push (:752-757) and species renew (:761-764) */
 int qp_sim_end_step(qp_sim *sim, double s);
 

@@ -34,6 +34,8 @@ struct qp_sim {
     qp_field2d *epw_hist;   // B_perp of the slice before the last (extrapolated seed)
     int hist_slices;        // slices of this step recorded in the history
     int prepared;           // slice whose first part has run (0: none)
+    int last_passes;        // corrector passes of the last finished slice
+    bool last_adaptive;     // whether its residuals were read back
 };
 
 using namespace qp;
@@ -273,8 +275,22 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     // slice j+1 of the 3-D wake (:731-735)
     RC(fused_slice_write(fxy, psit, bxyz, psi, s->bexyz, s->bbxyz, j + 1, dex, dd));
     for (qp_species *sp : s->sp) RC(species_check_post(sp));
+    s->last_passes = l;
+    s->last_adaptive = adaptive;
     if (iters_used) *iters_used = l;
     if (residual) *residual = adaptive ? res : -1.0;
+    return QP_OK;
+}
+
+int qp_slice_residuals(const qp_sim *s, double *res, int max, int *n)
+{
+    QP_REQUIRE(s && res && n && max >= 0, QP_ERR_ARG, "qp_slice_residuals: bad argument");
+    QP_REQUIRE(s->last_adaptive, QP_ERR_STATE,
+               "qp_slice_residuals: the last slice ran without a tolerance");
+    const int m = s->last_passes < RES_SLOTS - 1 ? s->last_passes : RES_SLOTS - 1;
+    *n = m < max ? m : max;
+    QP_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (int l = 1; l <= *n; ++l) res[l - 1] = s->ctx->h_res[l % RES_SLOTS];
     return QP_OK;
 }
 

@@ -278,6 +278,26 @@ module qpslice_c
        integer(c_int), intent(out) :: iters
        real(c_double), intent(out) :: res
      end function qp_slice_run
+     ! the slice in two parts, for a caller-provided corrector seed
+     integer(c_int) function qp_slice_prepare(s, j) bind(C, name="qp_slice_prepare")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: s
+       integer(c_int), value :: j
+     end function qp_slice_prepare
+     integer(c_int) function qp_slice_finish(s, j, iter_max, tol, iters, res) &
+          bind(C, name="qp_slice_finish")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: s
+       integer(c_int), value :: j, iter_max
+       real(c_double), value :: tol
+       integer(c_int), intent(out) :: iters
+       real(c_double), intent(out) :: res
+     end function qp_slice_finish
+     integer(c_int) function qp_sim_end_step(s, snew) bind(C, name="qp_sim_end_step")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: s
+       real(c_double), value :: snew
+     end function qp_sim_end_step
   end interface
 
 contains

@@ -513,6 +513,14 @@ class Sim:
               _D(tol), C.byref(it), C.byref(res))
         return it.value, res.value
 
+    def residuals(self):
+        """Corrector residuals of the last slice run with a tolerance, pass 1
+        first."""
+        a = np.zeros(64)
+        n = C.c_int()
+        _call(lib().qp_slice_residuals, self.h, _ptr(a), 64, C.byref(n))
+        return a[:n.value].copy()
+
     def end_step(self, s=0.0):
         _call(lib().qp_sim_end_step, self.h, _D(s))
 
