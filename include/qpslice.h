@@ -284,8 +284,8 @@ int qp_slice_run(qp_sim *sim, int j, int iter_max, double tol,
                  int *iters_used, double *residual);
 /* qp_slice_run in two parts, for a caller that sets the corrector's seed
  * itself (the learned seed): prepare runs the slice up to the seed (beam and
- * plasma charge, psi, E_perp, the psi gather, :650-670); finish runs the
- * rest.  Between them the caller may overwrite B_perp (QP_F_EPW, k space)
+ * plasma charge, psi, E_perp, :650-666, 670); finish runs the rest, the
+ * species' psi gather (:667-669) inside its first corrector pass.  Between them the caller may overwrite B_perp (QP_F_EPW, k space)
  * through qp_field2d_device; with QP_TUNE_SEED = 2 finish keeps it as the
  * seed.  No counterpart in the reference. */
 int qp_slice_prepare(qp_sim *sim, int j);

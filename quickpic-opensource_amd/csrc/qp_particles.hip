@@ -296,6 +296,22 @@ k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R d
     s.p6[i] = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
 }
 
+// WPGPSIPOST2L_QP for one particle of the cell at (fx, fy): p7 and p6 from
+// the psi gather, in k_extpsi's operation order (the first corrector pass
+// of a slice applies it just before amjdp_one)
+template <typename R>
+__device__ __forceinline__ void extpsi_one(const Corners<R> &f, R fx, R fy, R x, R y, R vx, R vy,
+                                           R qbm, R dex, R &p6, R &p7)
+{
+    const R dx2 = dex * dex;
+    R dxp = x - fx, dyp = y - fy;
+    R amx = R(1) - dxp, amy = R(1) - dyp;
+    R dx = gather(f, amx, amy, dxp, dyp);
+    dx = -(dx * dx2 * qbm);
+    p7 = R(1) + dx;
+    p6 = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
+}
+
 // the six fields amjdp and push gather, at the four corners of one cell
 template <typename R> struct Fields6 {
     Corners<R> ex, ey, ez, bx, by, bz;
@@ -378,7 +394,7 @@ template <typename R, int RG, int CH, int WPE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
         const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-        const int *skip)
+        const int *skip, const R *psi)
 {
     if (skip && *skip) return;
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
@@ -397,6 +413,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
         const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
 #pragma unroll
@@ -422,11 +439,18 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
             const int b = min(i1, c0 + n);
             for (int i = a; i < b; i += T) {
                 const int l = lpad(i - c0);
-                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], sp6[l], sp7[l],
-                                   sq[l], qtmh, qbm, dti, dex, idex, acc);
+                R p6 = sp6[l], p7 = sp7[l];
+                if (psi) {
+                    extpsi_one(FP, fx, fy, sx[l], sy[l], spx[l], spy[l], qbm, dex, p6, p7);
+                    sp7[l] = p7;
+                }
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l], qtmh,
+                                   qbm, dti, dex, idex, acc);
             }
             __syncthreads();
             for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
+            if (psi)
+                for (int p = tid; p < n; p += TCELLS) s.p7[c0 + p] = sp7[lpad(p)];
         }
 #pragma unroll
         for (int o = T / 2; o > 0; o >>= 1)
@@ -461,7 +485,7 @@ template <typename R, int RG, int CH>
 __global__ void __launch_bounds__(256)
 k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
            const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-           const int *skip)
+           const int *skip, const R *psi)
 {
     if (skip && *skip) return;
     constexpr int T = TILE / RG, NP = Stage<CH>::N, PF = CH / TCELLS;
@@ -501,6 +525,7 @@ k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, cons
         const int G2 = row + RG < TILE ? off[kb + (row + 2 * RG) * TILE] : G1;
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
         const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
 #pragma unroll
@@ -536,11 +561,18 @@ k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, cons
             const int b = min(i1, c0 + n);
             for (int i = a; i < b; i += T) {
                 const int l = lpad(i - c0);
-                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], sp6[l], sp7[l],
-                                   sq[l], qtmh, qbm, dti, dex, idex, acc);
+                R p6 = sp6[l], p7 = sp7[l];
+                if (psi) {
+                    extpsi_one(FP, fx, fy, sx[l], sy[l], spx[l], spy[l], qbm, dex, p6, p7);
+                    sp7[l] = p7;
+                }
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l], qtmh,
+                                   qbm, dti, dex, idex, acc);
             }
             __syncthreads();
             for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
+            if (psi)
+                for (int p = tid; p < n; p += TCELLS) s.p7[c0 + p] = sp7[lpad(p)];
         }
 #pragma unroll
         for (int o = T / 2; o > 0; o >>= 1)
@@ -582,7 +614,7 @@ template <typename R, int CHW>
 __global__ void __launch_bounds__(256)
 k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
           const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-          const int *skip)
+          const int *skip, const R *psi)
 {
     if (skip && *skip) return;
     constexpr int NP = Stage<CHW>::N, NR = 3 * 17;   // node rows of a row pair
@@ -605,6 +637,7 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
         const int G0 = off[kb + 2 * rp * TILE], G1 = off[kb + (2 * rp + 2) * TILE];
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
         const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
 #pragma unroll
@@ -630,11 +663,18 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
             const int b = min(i1, c0 + n);
             for (int i = a; i < b; i += 2) {
                 const int l = lpad(i - c0);
-                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], sp6[l], sp7[l],
-                                   sq[l], qtmh, qbm, dti, dex, idex, acc);
+                R p6 = sp6[l], p7 = sp7[l];
+                if (psi) {
+                    extpsi_one(FP, fx, fy, sx[l], sy[l], spx[l], spy[l], qbm, dex, p6, p7);
+                    sp7[l] = p7;
+                }
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l], qtmh,
+                                   qbm, dti, dex, idex, acc);
             }
             wave_sync();
             for (int p = lane; p < n; p += 64) s.p6[c0 + p] = sp6[lpad(p)];
+            if (psi)
+                for (int p = lane; p < n; p += 64) s.p7[c0 + p] = sp7[lpad(p)];
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -1159,7 +1199,8 @@ static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
 
 template <typename R, int V>
 static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
-                   const qp_field2d *psit, double dex, const DepOut<R> &out)
+                   const qp_field2d *psit, double dex, const DepOut<R> &out,
+                   const qp_field2d *psi)
 {
     qp_ctx *c = sp->ctx;
     const size_t pl = c->rplane;
@@ -1167,7 +1208,7 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     constexpr DepShape d = DepTab<R>::j[V];
     size_t lds;
     void (*kern)(TileGeom, Soa<R>, const int *, const R *, const R *, const R *, size_t,
-                 DepOut<R>, R *, size_t, R, R, R, const int *);
+                 DepOut<R>, R *, size_t, R, R, R, const int *, const R *);
     if constexpr (d.wave == 1) {
         lds = (4 * 7 * (size_t)Stage<d.ch>::N + 8 * 8 * 51) * sizeof(R);
         kern = k_amjdp_w<R, d.ch>;
@@ -1181,11 +1222,13 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
     {
-    ProfScope _ps(c, QP_PROF_AMJDP, (8.0 * sp->np + 14.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
+    // with psi: one more word per particle (p7 written) and one more plane
+    const double words = psi ? 9.0 : 8.0, planes = psi ? 15.0 : 14.0;
+    ProfScope _ps(c, QP_PROF_AMJDP, (words * sp->np + planes * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
                        psit->rs.as<R>(), pl, out, c->halo.as<R>(), hcs, (R)sp->qbm, (R)sp->dt,
-                       (R)dex, c->skip);
+                       (R)dex, c->skip, psi ? psi->rs.as<R>() : nullptr);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -1193,20 +1236,21 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
 
 template <typename R>
 static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
-                     const qp_field2d *psit, double dex, const DepOut<R> &out)
+                     const qp_field2d *psit, double dex, const DepOut<R> &out,
+                     const qp_field2d *psi = nullptr)
 {
     int rc;
     switch (sp->ctx->tune[QP_TUNE_AMJDP]) {
-    case 1: rc = amjdp_v<R, 1>(sp, ef, bf, psit, dex, out); break;
-    case 2: rc = amjdp_v<R, 2>(sp, ef, bf, psit, dex, out); break;
-    case 3: rc = amjdp_v<R, 3>(sp, ef, bf, psit, dex, out); break;
-    case 4: rc = amjdp_v<R, 4>(sp, ef, bf, psit, dex, out); break;
-    case 5: rc = amjdp_v<R, 5>(sp, ef, bf, psit, dex, out); break;
-    case 6: rc = amjdp_v<R, 6>(sp, ef, bf, psit, dex, out); break;
-    case 7: rc = amjdp_v<R, 7>(sp, ef, bf, psit, dex, out); break;
-    case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out); break;
-    case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out); break;
-    default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out); break;
+    case 1: rc = amjdp_v<R, 1>(sp, ef, bf, psit, dex, out, psi); break;
+    case 2: rc = amjdp_v<R, 2>(sp, ef, bf, psit, dex, out, psi); break;
+    case 3: rc = amjdp_v<R, 3>(sp, ef, bf, psit, dex, out, psi); break;
+    case 4: rc = amjdp_v<R, 4>(sp, ef, bf, psit, dex, out, psi); break;
+    case 5: rc = amjdp_v<R, 5>(sp, ef, bf, psit, dex, out, psi); break;
+    case 6: rc = amjdp_v<R, 6>(sp, ef, bf, psit, dex, out, psi); break;
+    case 7: rc = amjdp_v<R, 7>(sp, ef, bf, psit, dex, out, psi); break;
+    case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out, psi); break;
+    case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
+    default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;
     return deposit_border<R>(sp->ctx, out, 8, 1);
@@ -1553,7 +1597,8 @@ int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first)
 template <typename R>
 static int amjdp_fused_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                          const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu,
-                         qp_field2d *amu, double dex, bool first, bool last)
+                         qp_field2d *amu, double dex, bool first, bool last,
+                         const qp_field2d *psi)
 {
     const size_t pl = sp->ctx->rplane;
     R *c = cu->rs.as<R>(), *d = dcu->rs.as<R>(), *a = amu->rs.as<R>();
@@ -1566,7 +1611,7 @@ static int amjdp_fused_t(qp_species *sp, const qp_field2d *ef, const qp_field2d 
     o.p2[2] = sp->cu->rs.as<R>() + 2 * pl;
     o.scale = last ? (R)dex : R(1);
     o.scale2 = (R)dex;
-    int rc = amjdp_out<R>(sp, ef, bf, psit, dex, o);
+    int rc = amjdp_out<R>(sp, ef, bf, psit, dex, o, psi);
     sp->cu->state = 0;
     cu->state = dcu->state = amu->state = 0;
     return rc;
@@ -1574,8 +1619,9 @@ static int amjdp_fused_t(qp_species *sp, const qp_field2d *ef, const qp_field2d 
 
 int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
-                        double dex, bool first, bool last)
+                        double dex, bool first, bool last, const qp_field2d *psi)
 {
-    return DISPATCH(sp->ctx, amjdp_fused_t, sp, ef, bf, psit, cu, dcu, amu, dex, first, last);
+    return DISPATCH(sp->ctx, amjdp_fused_t, sp, ef, bf, psit, cu, dcu, amu, dex, first, last,
+                    psi);
 }
 }  // namespace qp

@@ -20,7 +20,7 @@ int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first);
 int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice);
 int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
-                        double dex, bool first, bool last);
+                        double dex, bool first, bool last, const qp_field2d *psi);
 }  // namespace qp
 
 struct qp_sim {
@@ -170,7 +170,7 @@ int qp_slice_prepare(qp_sim *s, int j)
         RC(fft_k2r_batch(c, fs, kinds, 2));
         fxy->state = psi->state = 0;
     }
-    for (qp_species *sp : s->sp) RC(qp_species_extpsi(sp, psi, dex));
+    // the species' psi gather (:667-669) runs inside the first corrector pass
     s->prepared = j;
     return QP_OK;
 }
@@ -214,9 +214,11 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
             RC(qp_field2d_as(dcu, 0.0));
             RC(qp_field2d_as(amu, 0.0));
         }
+        // the first pass applies the species' psi gather (extpsi, :667-669)
+        // to each particle just before its deposit
         for (int i = 0; i < nsp; ++i)
             RC(species_amjdp_fused(s->sp[i], fxy, bxyz, psit, cu, dcu, amu, dex, i == 0,
-                                   i == nsp - 1));
+                                   i == nsp - 1, l == 1 ? psi : nullptr));
         RC(fft_r2k_batch(c, fwd, fwd_kinds, 3));
         cu->state = dcu->state = amu->state = 1;
         RC(fused_pass(pf, dex, true, l, tol));
