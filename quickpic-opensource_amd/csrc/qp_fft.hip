@@ -115,11 +115,36 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     const bool live = ra < nrows, hb = ra + 1 < nrows;
     cplx<R> *buf = reinterpret_cast<cplx<R> *>(smem_raw) + f * S::LDSM;
 
-    // twiddles of every pass after the first, all fetched up front: pass p
-    // (Ns = 8^p) needs W^(r e), e = (t mod Ns) M / (8 Ns), r = 1..7: W^e,
-    // W^2e and W^4e come from the table, the other four are their products;
-    // the closing radix-4 pass needs W^j, W^2j (table) and W^3j, the
-    // closing radix-2 pass W^j
+    // first pass: radix 8 on the extension, straight from memory.  The 16
+    // data loads are issued first and unconditionally (a missing second row
+    // re-reads the first, a block past the last row reads row 0; the values
+    // are then discarded), so one memory latency covers them and the twiddle
+    // loads issued after them.
+    R la[8], lb[8];
+    {
+        const R *base = cs.src[comp];
+        const R *sa = live ? base + (COL == 1 ? (size_t)ra : (size_t)ra * ld_src) : base;
+        const R *sb = (COL == 0 && live && hb) ? sa + ld_src : sa;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int idx = t + r * TF;
+            const int e = idx <= n ? idx : M - idx;
+            if (COL == 1) {
+                // the pair of columns ra, ra+1 of row e (ra even, ld_src even)
+                const cplx<R> p2 = *reinterpret_cast<const cplx<R> *>(sa + (size_t)e * ld_src);
+                la[r] = p2.x;
+                lb[r] = p2.y;
+            } else {
+                la[r] = sa[e];
+                lb[r] = sb[e];
+            }
+        }
+    }
+
+    // twiddles of every pass after the first: pass p (Ns = 8^p) needs
+    // W^(r e), e = (t mod Ns) M / (8 Ns), r = 1..7: W^e, W^2e and W^4e come
+    // from the table, the other four are their products; the closing radix-4
+    // pass needs W^j, W^2j (table) and W^3j, the closing radix-2 pass W^j
     constexpr int NTW8 = S::NR8 > 1 ? 3 * (S::NR8 - 1) : 1;
     cplx<R> tw8[NTW8];
 #pragma unroll
@@ -143,35 +168,22 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
         for (int s = 0; s < 4; ++s) twr[s] = W[t + s * TF];
     }
 
-    // first pass: radix 8 on the extension, straight from memory
     cplx<R> v[8];
-    {
-        const R *sa = cs.src[comp] + (COL == 1 ? (size_t)ra : (size_t)ra * ld_src);
-        const R *sb = sa + ld_src;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int idx = t + r * TF;
-            const int e = idx <= n ? idx : M - idx;
-            R va = R(0), vb = R(0);
-            if (COL == 1) {
-                if (live) {
-                    // the pair of columns ra, ra+1 of row e (ra even, ld_src even)
-                    const cplx<R> p2 = *reinterpret_cast<const cplx<R> *>(sa + (size_t)e * ld_src);
-                    va = p2.x;
-                    vb = hb ? p2.y : R(0);
-                }
-            } else if (live) {
-                va = sa[e];
-                if (hb) vb = sb[e];
-            }
-            if (type == 0) {
-                if (e == 0 || e == n) { va = R(0); vb = R(0); }
-                if (idx > n) { va = -va; vb = -vb; }
-            }
-            v[r] = {va, vb};
+    for (int r = 0; r < 8; ++r) {
+        const int idx = t + r * TF;
+        const int e = idx <= n ? idx : M - idx;
+        R va = live ? la[r] : R(0);
+        R vb = (live && hb) ? lb[r] : R(0);
+        if (type == 0) {
+            const bool edge = e == 0 || e == n;
+            const R sg = idx > n ? R(-1) : R(1);
+            va = edge ? R(0) : sg * va;
+            vb = edge ? R(0) : sg * vb;
         }
-        dft8(v);
+        v[r] = {va, vb};
     }
+    dft8(v);
 
     const R half = scale * R(0.5);
     R *da = cs.dst[comp] + (COL == 2 ? (size_t)ra : (size_t)ra * ld_dst);

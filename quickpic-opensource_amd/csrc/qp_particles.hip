@@ -188,6 +188,33 @@ template <int CH> struct Stage {
     static constexpr int N = CH + CH / 32 + 1;
 };
 
+// Stages particles [c0, c0 + n) of NA arrays into LDS (padded index lpad):
+// thread tid takes p = tid + k*step, k < K.  Every load of the thread is
+// issued before the first LDS store, so the chunk costs one memory latency.
+template <typename R, int NA, int K>
+__device__ __forceinline__ void stage_in(const R *const (&src)[NA], R *const (&dst)[NA], int c0,
+                                         int n, int tid, int step)
+{
+    R t[K][NA];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int p = tid + k * step;
+        if (p < n) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) t[k][a] = src[a][c0 + p];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int p = tid + k * step;
+        if (p < n) {
+            const int l = lpad(p);
+#pragma unroll
+            for (int a = 0; a < NA; ++a) dst[a][l] = t[k][a];
+        }
+    }
+}
+
 // Interior nodes of the tile's 17x17 block to the output plane, border nodes
 // to the halo buffer (summed over tiles by k_deposit_border).
 template <typename R>
@@ -235,11 +262,10 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R 
         for (int c0 = G0; c0 < G1; c0 += CH) {
             const int n = min(CH, G1 - c0);
             __syncthreads();
-            for (int p = tid; p < n; p += TCELLS) {
-                const int l = lpad(p);
-                sx[l] = s.x[c0 + p];
-                sy[l] = s.y[c0 + p];
-                sq[l] = s.q[c0 + p];
+            {
+                const R *src[3] = {s.x, s.y, s.q};
+                R *dst[3] = {sx, sy, sq};
+                stage_in<R, 3, CH / TCELLS>(src, dst, c0, n, tid, TCELLS);
             }
             __syncthreads();
             int a = i0 + sub;
@@ -423,15 +449,10 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         for (int c0 = G0; c0 < G1; c0 += CH) {
             const int n = min(CH, G1 - c0);
             __syncthreads();
-            for (int p = tid; p < n; p += TCELLS) {
-                const int l = lpad(p), q = c0 + p;
-                sx[l] = s.x[q];
-                sy[l] = s.y[q];
-                spx[l] = s.px[q];
-                spy[l] = s.py[q];
-                sp6[l] = s.p6[q];
-                sp7[l] = s.p7[q];
-                sq[l] = s.q[q];
+            {
+                const R *src[7] = {s.x, s.y, s.px, s.py, s.p6, s.p7, s.q};
+                R *dst[7] = {sx, sy, spx, spy, sp6, sp7, sq};
+                stage_in<R, 7, CH / TCELLS>(src, dst, c0, n, tid, TCELLS);
             }
             __syncthreads();
             int a = i0 + sub;
@@ -647,15 +668,10 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
         for (int c0 = G0; c0 < G1; c0 += CHW) {
             const int n = min(CHW, G1 - c0);
             wave_sync();
-            for (int p = lane; p < n; p += 64) {
-                const int l = lpad(p), q = c0 + p;
-                sx[l] = s.x[q];
-                sy[l] = s.y[q];
-                spx[l] = s.px[q];
-                spy[l] = s.py[q];
-                sp6[l] = s.p6[q];
-                sp7[l] = s.p7[q];
-                sq[l] = s.q[q];
+            {
+                const R *src[7] = {s.x, s.y, s.px, s.py, s.p6, s.p7, s.q};
+                R *dst[7] = {sx, sy, spx, spy, sp6, sp7, sq};
+                stage_in<R, 7, CHW / 64>(src, dst, c0, n, lane, 64);
             }
             wave_sync();
             int a = i0 + sub;
@@ -732,10 +748,48 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
        const R *psit, size_t cs, R qbm, R dt, R dex, int *bin, int *rank, int *cnt, int *err)
 {
     __shared__ int hist[NBINS];
+    __shared__ R fnode[6][289];   // the six fields at the tile's 17 x 17 nodes
     const int S = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < NBINS; k += TCELLS) hist[k] = 0;
     const int tx0 = S % g.mx1, ty0 = S / g.mx1;
     const int p0 = off[S * TCELLS], p1 = off[(S + 1) * TCELLS];
+    // the particle record of chunk c0 is loaded while chunk c0 - 256 is ranked
+    R nx_ = R(0), ny_ = R(0), npx = R(0), npy = R(0), npz = R(0), np6 = R(1), np7 = R(1);
+    auto fetch = [&](int c) {
+        const int i = c + tid;
+        if (i < p1) {
+            nx_ = s.x[i];
+            ny_ = s.y[i];
+            npx = s.px[i];
+            npy = s.py[i];
+            npz = s.pz[i];
+            np6 = s.p6[i];
+            np7 = s.p7[i];
+        }
+    };
+    fetch(p0);
+    // every particle of the tile lies in one of its cells (the sort put it
+    // there), so the corners it gathers are tile nodes: read them once
+    {
+        constexpr int NF = (6 * 289 + TCELLS - 1) / TCELLS;
+        R tv[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const int n = tid + k * TCELLS;
+            if (n < 6 * 289) {
+                const int f = n / 289, m = n % 289;
+                const R *pl = f == 0 ? fxy : f == 1 ? fxy + cs : f == 2 ? psit
+                            : f == 3 ? bxy : f == 4 ? bxy + cs : bxy + 2 * cs;
+                tv[k] = pl[(size_t)(ty0 * TILE + m / 17) * g.ldr + tx0 * TILE + m % 17];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const int n = tid + k * TCELLS;
+            if (n < 6 * 289) fnode[n / 289][n % 289] = tv[k];
+        }
+    }
+    __syncthreads();
     const R idex = R(1) / dex, qtmh = R(0.5) * qbm * dt;
     const R anx = (R)g.nx, any = (R)g.ny;
     const R edgelx = (R)(tx0 * TILE), edgely = (R)(ty0 * TILE);
@@ -744,17 +798,20 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
     for (int c0 = p0; c0 < p1; c0 += TCELLS) {
         const int i = c0 + tid;
         const bool valid = i < p1;
+        const R x = nx_, y = ny_, ppx = npx, ppy = npy, ppz = npz, p6 = np6, p7 = np7;
+        if (c0 + TCELLS < p1) fetch(c0 + TCELLS);
         int b = 0;
         if (valid) {
-            R x = s.x[i], y = s.y[i];
             const int cx = min(max((int)x, 0), g.nx - 1), cy = min(max((int)y, 0), g.ny - 1);
-            const Corners<R> ex = load_corners(fxy, g.ldr, cx, cy);
-            const Corners<R> ey = load_corners(fxy + cs, g.ldr, cx, cy);
-            const Corners<R> ez = load_corners(psit, g.ldr, cx, cy);
-            const Corners<R> bx = load_corners(bxy, g.ldr, cx, cy);
-            const Corners<R> by = load_corners(bxy + cs, g.ldr, cx, cy);
-            const Corners<R> bz = load_corners(bxy + 2 * cs, g.ldr, cx, cy);
-            R p6 = s.p6[i], p7 = s.p7[i];
+            // local node of the cell, clamped into the tile (a particle flagged
+            // by the move check may sit outside it; its result is reported)
+            const int li = min(max(cx - tx0 * TILE, 0), TILE - 1);
+            const int lj = min(max(cy - ty0 * TILE, 0), TILE - 1);
+            const int m = lj * 17 + li;
+            auto lc = [&](int f) {
+                return Corners<R>{fnode[f][m], fnode[f][m + 1], fnode[f][m + 17], fnode[f][m + 18]};
+            };
+            const Corners<R> ex = lc(0), ey = lc(1), ez = lc(2), bx = lc(3), by = lc(4), bz = lc(5);
             R qtmh1 = qtmh / p7;
             R qtmh2 = qtmh1 * p6;
             R dxp = x - (R)cx, dyp = y - (R)cy;
@@ -771,7 +828,7 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
             dx = qtmh2 * dx;
             dy = qtmh2 * dy;
             dz = qtmh2 * dz;
-            R acx = s.px[i] + dx, acy = s.py[i] + dy, acz = s.pz[i] + dz;
+            R acx = ppx + dx, acy = ppy + dy, acz = ppz + dz;
             R omxt = qtmh1 * ox, omyt = qtmh1 * oy, omzt = qtmh1 * oz;
             R omt = omxt * omxt + omyt * omyt + omzt * omzt;
             R anorm = R(2) / (R(1) + omt);
@@ -1049,7 +1106,7 @@ template <typename R> struct DepTab;
 template <> struct DepTab<float> {
     static constexpr DepShape q[] = {{16, 1024, 1, 0}, {16, 4096, 1, 0}, {8, 2048, 1, 0},
                                      {16, 2048, 1, 0}};
-    static constexpr DepShape j[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {4, 1024, 4, 0},
+    static constexpr DepShape j[] = {{8, 1024, 1, 0}, {8, 1024, 4, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
                                      {8, 512, 1, 2}};
