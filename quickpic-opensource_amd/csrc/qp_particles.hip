@@ -943,18 +943,24 @@ __global__ void k_scatter(int64_t np, Soa<R> a, Soa<R> b, const int *bin, const 
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= np) return;
-    const int bi = bin[i];
+    // the record is read completely, and first, before any store (the
+    // compiler may not move a load of a past a store to b, which it cannot
+    // prove disjoint); its loads overlap those of the destination
+    const R v0 = a.x[i], v1 = a.y[i], v2 = a.px[i], v3 = a.py[i], v4 = a.pz[i], v5 = a.p6[i],
+            v6 = a.p7[i], v7 = a.q[i];
+    __builtin_amdgcn_sched_barrier(0);   // keep these loads ahead of the lookups
+    const int bi = bin[i], rk = rank[i];
     const int S = bi / NBINS, bb = bi - S * NBINS, nb = bb >> 8, cl = bb & 255;
     const int tx = (S % mx1 + nb % 3 - 1 + mx1) % mx1, ty = (S / mx1 + nb / 3 - 1 + my1) % my1;
-    const int d = off[(ty * mx1 + tx) * TCELLS + cl] + cnt[bi] + rank[i];
-    b.x[d] = a.x[i];
-    b.y[d] = a.y[i];
-    b.px[d] = a.px[i];
-    b.py[d] = a.py[i];
-    b.pz[d] = a.pz[i];
-    b.p6[d] = a.p6[i];
-    b.p7[d] = a.p7[i];
-    b.q[d] = a.q[i];
+    const int d = off[(ty * mx1 + tx) * TCELLS + cl] + cnt[bi] + rk;
+    b.x[d] = v0;
+    b.y[d] = v1;
+    b.px[d] = v2;
+    b.py[d] = v3;
+    b.pz[d] = v4;
+    b.p6[d] = v5;
+    b.p7[d] = v6;
+    b.q[d] = v7;
 }
 
 // cell keys of unsorted particles (initial load and uploads), with their
@@ -985,14 +991,16 @@ __global__ void k_gather(int64_t np, Soa<R> a, Soa<R> b, const int *src)
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int i = src[p];
-    b.x[p] = a.x[i];
-    b.y[p] = a.y[i];
-    b.px[p] = a.px[i];
-    b.py[p] = a.py[i];
-    b.pz[p] = a.pz[i];
-    b.p6[p] = a.p6[i];
-    b.p7[p] = a.p7[i];
-    b.q[p] = a.q[i];
+    const R v0 = a.x[i], v1 = a.y[i], v2 = a.px[i], v3 = a.py[i], v4 = a.pz[i], v5 = a.p6[i],
+            v6 = a.p7[i], v7 = a.q[i];
+    b.x[p] = v0;
+    b.y[p] = v1;
+    b.px[p] = v2;
+    b.py[p] = v3;
+    b.pz[p] = v4;
+    b.p6[p] = v5;
+    b.p7[p] = v6;
+    b.q[p] = v7;
 }
 
 __global__ void k_zero_int(int *p, int n)
