@@ -259,8 +259,19 @@ class UNet:
     def forward_graph(self, planes, plane, ld, H, W):
         """forward() replayed from a HIP graph captured on the first call
         with these buffers and sizes (the weights are read at replay, so
-        training between replays is seen); returns the output buffer."""
-        key = (planes.data_ptr(), plane, ld, H, W, torch.cuda.current_stream().cuda_stream)
+        training between replays is seen); returns the output buffer.  The
+        legacy null stream cannot be captured: from it, the graph runs on a
+        side stream ordered after and before the caller's work."""
+        cur = torch.cuda.current_stream()
+        if cur.cuda_stream == 0:
+            if getattr(self, "_side", None) is None:
+                self._side = torch.cuda.Stream()
+            self._side.wait_stream(cur)
+            with torch.cuda.stream(self._side):
+                out = self.forward_graph(planes, plane, ld, H, W)
+            cur.wait_stream(self._side)
+            return out
+        key = (planes.data_ptr(), plane, ld, H, W, cur.cuda_stream)
         if self._graph is None or self._graph[0] != key:
             self.drop_graph()
             self.forward(planes, plane, ld, H, W)   # buffers allocated outside the capture
