@@ -124,7 +124,8 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     {
         const R *base = cs.src[comp];
         const R *sa = live ? base + (COL == 1 ? (size_t)ra : (size_t)ra * ld_src) : base;
-        const R *sb = (COL == 0 && live && hb) ? sa + ld_src : sa;
+        // rows are read as rows except by the column-pair reader (COL = 1)
+        const R *sb = (COL != 1 && live && hb) ? sa + ld_src : sa;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int idx = t + r * TF;
