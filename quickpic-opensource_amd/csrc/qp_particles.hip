@@ -1114,7 +1114,7 @@ template <typename R> struct DepTab;
 template <> struct DepTab<float> {
     static constexpr DepShape q[] = {{16, 1024, 1, 0}, {16, 4096, 1, 0}, {8, 2048, 1, 0},
                                      {16, 2048, 1, 0}};
-    static constexpr DepShape j[] = {{8, 1024, 1, 0}, {8, 1024, 4, 0}, {4, 1024, 4, 0},
+    static constexpr DepShape j[] = {{8, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
                                      {8, 512, 1, 2}};
