@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "qp_common.h"
 
@@ -352,6 +353,32 @@ __device__ __forceinline__ Fields6<R> load_fields6(const R *fxy, const R *bxy, c
                       load_corners(bxy + cs, ld, cx, cy), load_corners(bxy + 2 * cs, ld, cx, cy)};
 }
 
+// FP32: a cell's corner values rewritten once as bilinear coefficients,
+// f(dxp, dyp) = c0 + dxp c1 + dyp (c2 + dxp c3), so each gather of the cell's
+// particles is three FMAs; FP64 keeps the corner values and the reference's
+// operation order (gather above)
+template <typename R>
+__device__ __forceinline__ void prep_fields(Fields6<R> &F)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        Corners<R> *c[6] = {&F.ex, &F.ey, &F.ez, &F.bx, &F.by, &F.bz};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const R f00 = c[k]->f00, f10 = c[k]->f10, f01 = c[k]->f01, f11 = c[k]->f11;
+            *c[k] = Corners<R>{f00, f10 - f00, f01 - f00, (f11 - f10) - (f01 - f00)};
+        }
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R cell_gather(const Corners<R> &f, R amx, R amy, R dxp, R dyp)
+{
+    if constexpr (std::is_same<R, float>::value)
+        return fmaf(dyp, fmaf(dxp, f.f11, f.f01), fmaf(dxp, f.f10, f.f00));
+    else
+        return gather(f, amx, amy, dxp, dyp);
+}
+
 // One particle of PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768), in the
 // reference's operation order: gather, half Boris step with the B_z
 // rotation, the 8 deposited quantities times the 4 corner weights added to
@@ -366,12 +393,12 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
     R inv_part_7 = recip(p7);
     R qtmh1 = qtmh * inv_part_7;
     R qtmh2 = qtmh1 * p6;
-    R dx = gather(F.ex, amx, amy, dxp, dyp);
-    R dy = gather(F.ey, amx, amy, dxp, dyp);
-    R dz = gather(F.ez, amx, amy, dxp, dyp);
-    R ox = gather(F.bx, amx, amy, dxp, dyp);
-    R oy = gather(F.by, amx, amy, dxp, dyp);
-    R oz = gather(F.bz, amx, amy, dxp, dyp);
+    R dx = cell_gather(F.ex, amx, amy, dxp, dyp);
+    R dy = cell_gather(F.ey, amx, amy, dxp, dyp);
+    R dz = cell_gather(F.ez, amx, amy, dxp, dyp);
+    R ox = cell_gather(F.bx, amx, amy, dxp, dyp);
+    R oy = cell_gather(F.by, amx, amy, dxp, dyp);
+    R oz = cell_gather(F.bz, amx, amy, dxp, dyp);
     R ddx = (R(-1)) * qtmh2 * dx + qtmh * oy;
     R ddy = (R(-1)) * qtmh2 * dy - qtmh * ox;
     R vx = px, vy = py;
@@ -438,7 +465,8 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         const int i0 = off[key], i1 = off[key + 1];
         const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
-        const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
@@ -545,7 +573,8 @@ k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, cons
         const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
         const int G2 = row + RG < TILE ? off[kb + (row + 2 * RG) * TILE] : G1;
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
-        const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
@@ -657,7 +686,8 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
         const int i0 = off[key], i1 = off[key + 1];
         const int G0 = off[kb + 2 * rp * TILE], G1 = off[kb + (2 * rp + 2) * TILE];
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
-        const Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
         const R fx = (R)cx, fy = (R)cy;
         R acc[8][4];
