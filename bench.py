@@ -58,6 +58,9 @@ def run_gpu(args, precision, beam_part=None):
     import qpslice as qps
     ind = args.ind
     ctx = qps.Ctx(ind, ind, args.indz, precision=precision)
+    # corrector seed: the FP32 leg uses --fp32-seed, the plain FP64 leg the
+    # reference's previous-slice field; both iterate to the same tolerance
+    ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
     n = ctx.nx
     t0 = time.time()
     sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
@@ -138,7 +141,7 @@ def run_gpu(args, precision, beam_part=None):
             sweep[f"seed[{v}] ms/slice"] = round((time.perf_counter() - t0) / 10 * 1e3, 4)
             sweep[f"seed[{v}] passes"] = float(np.mean(its[2:]))
             j0 += 10
-        ctx.tune(qps.TUNE_SEED, -1)
+        ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
         # transforms: transpose kernels / column-pair y pass
         for v in (0, 1):
             ctx.tune(qps.TUNE_FFT, v)
@@ -265,6 +268,8 @@ def main():
     ap.add_argument("--beam-np", type=int, nargs=3, default=[128, 128, 2048])
     ap.add_argument("--peak", type=float, default=10.0)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--fp32-seed", type=int, default=1, choices=(0, 1),
+                    help="corrector seed of the FP32 leg: 0 previous slice, 1 extrapolated")
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")
@@ -329,7 +334,10 @@ def main():
             "workload": f"precision form: {2 ** args.ind}^2 x {2 ** args.indz} slices, "
                         f"{args.ppc * args.ppc} ppc, beam {r32['np_beam']} particles, "
                         f"nb/n0={args.peak}, B residual {args.tol:g} (max {args.iter_max} "
-                        "passes), plain seed (no CNN)",
+                        "passes), FP32 corrector seed: "
+                        + {0: "previous slice", 1: "linear extrapolation"}[args.fp32_seed]
+                        + "; FP64 leg: previous slice (plain)",
+            "fp32_seed": {0: "previous", 1: "extrapolated"}[args.fp32_seed],
             "grid": [2 ** args.ind, 2 ** args.ind, 2 ** args.indz],
             "ppc": args.ppc * args.ppc,
             "plasma_particles": r32["np_plasma"],
