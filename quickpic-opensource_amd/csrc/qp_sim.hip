@@ -153,16 +153,22 @@ int qp_slice_prepare(qp_sim *s, int j)
     const int nsp = (int)s->sp.size();
     // the move check of an earlier slice, if it has arrived
     for (qp_species *sp : s->sp) RC(species_check_poll(sp, false));
-    // beam charge and its transverse field (:650-656)
+    // beam charge (:650-655) and plasma charge (:657-661); the two forward
+    // transforms are independent and run as one batch
     if (s->bm.empty()) RC(qp_field2d_as(qb, 0.0));
     for (size_t i = 0; i < s->bm.size(); ++i)
         RC(i == 0 ? beam_slice_assign(s->bm[0], qb, j + 1) : qp_beam_qdpcopy(s->bm[i], qb, j + 1));
-    RC(qp_field2d_fftrk(qb, 1));
-    RC(qp_field2d_elf(qb, epwb));
-    // plasma charge, psi and E_perp (:657-670)
     if (nsp == 0) RC(qp_field2d_as(qe, 0.0));
     for (int i = 0; i < nsp; ++i) RC(species_qdp_fused(s->sp[i], qe, i == 0));
-    RC(qp_field2d_fftrk(qe, 1));
+    QP_REQUIRE(qb->state == 0 && qe->state == 0, QP_ERR_STATE, "fftrk: data is in ks");
+    {
+        qp_field2d *fs[2] = {qb, qe};
+        const int kinds[2] = {1, 1};
+        RC(fft_r2k_batch(c, fs, kinds, 2));
+        qb->state = qe->state = 1;
+    }
+    // the beam's transverse field (:656), psi and E_perp (:662-666)
+    RC(qp_field2d_elf(qb, epwb));
     RC(fused_psi_fxy(qe, psi, fxy));
     {
         qp_field2d *fs[2] = {fxy, psi};
