@@ -238,11 +238,35 @@ typedef struct {
     int64_t npmax;
 } qp_beam_params;
 
+/* Beam profile 2 (fdist3d_002, fdist3d_class.f03:652-821): transverse phase
+ * space from Twiss parameters, Gaussian in z, loaded by the algorithm of
+ * PRVDIST32_TWISS (part3d_lib77.f:2-192): cut-off at 3 sigma in each
+ * coordinate and at radius 3 in the normalised transverse pair, momentum
+ * p_x = emit_x/sigma_x r - gamma alpha_x/beta_x (x - x0), p_z = gamma +
+ * sigma_vz r.  Lengths in c/wp as in the input deck (beta, emittance, centre,
+ * sigma_z); sigma_x = sqrt(beta_x emittance_x / gamma). */
+typedef struct {
+    double center[3];      /* c/wp, in box coordinates */
+    double alpha[2];       /* Twiss alpha, x and y */
+    double beta[2];        /* Twiss beta, x and y (c/wp) */
+    double emittance[2];   /* normalised emittance, x and y (c/wp) */
+    double sigma_z;        /* c/wp */
+    double sigma_vz;       /* mc */
+    double gamma, peak, q, m;
+    int np[3];             /* npx, npy, npz */
+    int quiet;
+    int evolution;
+    int64_t npmax;
+} qp_beam_twiss_params;
+
 /* init_beam3d :61-93; seed 0 is the reference's fixed ranorm state,
  * other seeds start independent streams.  np = {0,0,0} with npmax > 0 makes
  * an empty beam of that capacity, to be filled by qp_beam_upload. */
 int qp_beam_create(qp_ctx *ctx, const qp_beam_params *p, double dt,
                    uint64_t seed, qp_beam **out);
+/* init_fdist3d_002 + dist3d_002 (fdist3d_class.f03:652-821); seeds as above */
+int qp_beam_create_twiss(qp_ctx *ctx, const qp_beam_twiss_params *p, double dt,
+                         uint64_t seed, qp_beam **out);
 int qp_beam_destroy(qp_beam *b);
 int qp_beam_qdp(qp_beam *b);                                       /* :111 */
 int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice);         /* :136 */

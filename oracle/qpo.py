@@ -179,6 +179,24 @@ class BeamSpec:
 
 
 @dataclass
+class TwissSpec:
+    """Beam profile 2 (fdist3d_class.f03:652-821): Twiss parameters in x, y,
+    Gaussian in z.  Lengths in c/wp as in the input deck."""
+    center: tuple = (0.0, 0.0, 2.0)
+    alpha: tuple = (0.0, 0.0)
+    beta: tuple = (1.0, 1.0)
+    emittance: tuple = (1e-4, 1e-4)
+    sigma_z: float = 0.5
+    sigma_vz: float = 0.0
+    np_: tuple = (32, 32, 64)
+    gamma: float = 2.0e4
+    peak: float = 4.0
+    q: float = -1.0
+    m: float = 1.0
+    quiet: bool = True
+
+
+@dataclass
 class SpeciesSpec:
     """Uniform plasma, profile 0 (fdist2d_class.f03:168-316)."""
     ppc: tuple = (2, 2)
@@ -203,6 +221,84 @@ def beam_params(grid: Grid, b: BeamSpec) -> BeamParams:
     bp.npx, bp.npy, bp.npz = b.np_
     bp.quiet = 1 if b.quiet else 0
     return bp
+
+
+def beam_init_twiss(grid: Grid, b: TwissSpec, npmax=None, rng=None):
+    """Profile 2: init_fdist3d_002 units and charge (fdist3d_class.f03:720-745)
+    and PRVDIST32_TWISS (part3d_lib77.f:2-192) with one partition covering
+    the box, drawing from the same ranorm stream.  Returns (particles (np,7),
+    qm, ierr).  Pure Python over the C ranorm: for small beams."""
+    import math
+    rng = rng or rng_new()
+    r = lambda: lib().qpo_ranorm(C.byref(rng))  # noqa: E731
+    dx, dy, dz = grid.dx, grid.dy, grid.dz
+    gamma = b.gamma
+    npx, npy, npz = b.np_
+    npmax = npmax or npx * npy * npz + 2
+    sxp = math.sqrt(b.beta[0] * b.emittance[0] / gamma)
+    syp = math.sqrt(b.beta[1] * b.emittance[1] / gamma)
+    qm = b.q / abs(b.q) * abs(b.peak) * (2 * 3.1415926535897932) ** 1.5 * sxp * syp * b.sigma_z
+    qm = qm / dx / dy / dz
+    qm = qm / npx
+    qm = qm / npy
+    qm = qm / npz
+    x0 = (b.center[0] - grid.box[0][0]) / dx
+    y0 = (b.center[1] - grid.box[1][0]) / dy
+    z0 = (b.center[2] - grid.box[2][0]) / dz
+    sigz, vtz, vdz = b.sigma_z / dz, b.sigma_vz, gamma
+    ax, ay = b.alpha
+    bx, by = b.beta[0] / dx, b.beta[1] / dy
+    ex, ey = b.emittance[0] / dx, b.emittance[1] / dy
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    sigx = math.sqrt(bx * ex / gamma)
+    sigy = math.sqrt(by * ey / gamma)
+    vtx, vty = ex / sigx, ey / sigy
+    blx, bly, blz = max(x0 - 3.0 * sigx, 1.0), max(y0 - 3.0 * sigy, 1.0), max(z0 - 3.0 * sigz, 1.0)
+    brx, bry = min(x0 + 3.0 * sigx, float(nx - 1)), min(y0 + 3.0 * sigy, float(ny - 1))
+    brz = min(z0 + 3.0 * sigz, float(nz - 1))
+    out, ierr = [], 0
+    j, l = 0, npz
+    while j < npx:
+        k = 0
+        while k < npy:
+            l -= npz
+            while l < npz:
+                while True:
+                    tz = z0 + sigz * r()
+                    if not (tz >= brz or tz <= blz):
+                        break
+                while True:                              # label 20
+                    while True:
+                        tx = r()
+                        px_ = x0 + sigx * tx
+                        if not (px_ >= brx or px_ <= blx):
+                            break
+                    while True:                          # label 30
+                        ty = r()
+                        py_ = y0 + sigy * ty
+                        if not (py_ >= bry or py_ <= bly):
+                            break
+                    if not (tx * tx + ty * ty > 9.0):
+                        break
+                vx = vtx * r()
+                vy = vty * r()
+                vz = vtz * r() + vdz
+                vx = vx - gamma * ax / bx * (px_ - x0)
+                vy = vy - gamma * ay / by * (py_ - y0)
+                if 0.0 <= py_ < ny and 0.0 <= tz < nz:
+                    for rec in ((px_, py_, tz, vx, vy, vz, qm),
+                                (2 * x0 - px_, 2 * y0 - py_, tz, -vx, -vy, vz, qm))[
+                                    : 2 if b.quiet else 1]:
+                        if len(out) + 1 < npmax:
+                            out.append(rec)
+                        else:
+                            ierr += 1
+                l += 1
+                if b.quiet:
+                    l += 1
+            k += 1
+        j += 1
+    return np.array(out, dtype=np.float64).reshape(-1, 7), qm, ierr
 
 
 def beam_init(grid: Grid, b: BeamSpec, npmax=None, rng=None):

@@ -181,6 +181,90 @@ static int64_t load_beam(qp_ctx *c, const qp_beam_params &bp, uint64_t seed,
     return npp;
 }
 
+// fdist3d_002 normalisation (fdist3d_class.f03:720-745) and PRVDIST32_TWISS
+// (part3d_lib77.f:2-192) with one partition covering the box.  Fills
+// h[7][n]; returns the particle count or -1 on overflow.
+static int64_t load_beam_twiss(qp_ctx *c, const qp_beam_twiss_params &bp, uint64_t seed,
+                               std::vector<double> (&h)[7], int64_t npmax, double *qm_out)
+{
+    const double dx = c->dex, dy = c->dey, dz = c->dxi;
+    const double gamma = bp.gamma;
+    // charge per particle from the physical sizes (:729-736)
+    const double sx_phys = std::sqrt(bp.beta[0] * bp.emittance[0] / gamma);
+    const double sy_phys = std::sqrt(bp.beta[1] * bp.emittance[1] / gamma);
+    double qm = bp.q / std::fabs(bp.q) * std::fabs(bp.peak) *
+                std::pow(2 * 3.1415926535897932, 1.5) * sx_phys * sy_phys * bp.sigma_z;
+    qm = qm / dx / dy / dz;
+    qm = qm / bp.np[0];
+    qm = qm / bp.np[1];
+    qm = qm / bp.np[2];
+    *qm_out = qm;
+    // grid units (:737-745)
+    const double x0 = (bp.center[0] - c->box[0]) / dx;
+    const double y0 = (bp.center[1] - c->box[2]) / dy;
+    const double z0 = (bp.center[2] - c->box[4]) / dz;
+    const double sigz = bp.sigma_z / dz, vtz = bp.sigma_vz, vdz = gamma;
+    const double alpha_x = bp.alpha[0], alpha_y = bp.alpha[1];
+    const double beta_x = bp.beta[0] / dx, beta_y = bp.beta[1] / dy;
+    const double emt_x = bp.emittance[0] / dx, emt_y = bp.emittance[1] / dy;
+    // PRVDIST32_TWISS (:62-110)
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const double x2 = 2.0 * x0, y2 = 2.0 * y0;
+    const double sigx = std::sqrt(beta_x * emt_x / gamma);
+    const double sigy = std::sqrt(beta_y * emt_y / gamma);
+    const double vtx = emt_x / sigx, vty = emt_y / sigy;
+    const double blx = std::fmax(x0 - 3.0 * sigx, 1.0), bly = std::fmax(y0 - 3.0 * sigy, 1.0);
+    const double blz = std::fmax(z0 - 3.0 * sigz, 1.0);
+    const double brx = std::fmin(x0 + 3.0 * sigx, (double)(nx - 1));
+    const double bry = std::fmin(y0 + 3.0 * sigy, (double)(ny - 1));
+    const double brz = std::fmin(z0 + 3.0 * sigz, (double)(nz - 1));
+    Ranorm rng(seed);
+    int64_t npp = 0, npt = 1;
+    int j = 0, k, l = bp.np[2];
+    auto put = [&](double x, double y, double z, double px, double py, double pz) {
+        npt = npp + 1;
+        size_t i = (size_t)(npt - 1);
+        h[0][i] = x; h[1][i] = y; h[2][i] = z;
+        h[3][i] = px; h[4][i] = py; h[5][i] = pz; h[6][i] = qm;
+        npp = npt;
+    };
+    while (j < bp.np[0]) {
+        k = 0;
+        while (k < bp.np[1]) {
+            l = l - bp.np[2];
+            while (l < bp.np[2]) {
+                double tz, tx, ty, rx, ry;
+                do { tz = z0 + sigz * rng.next(); } while (tz >= brz || tz <= blz);
+                // x then y, each within its 3-sigma window, redrawn together
+                // while the pair lies outside radius 3 (:98-106)
+                for (;;) {
+                    do { rx = rng.next(); tx = x0 + sigx * rx; } while (tx >= brx || tx <= blx);
+                    do { ry = rng.next(); ty = y0 + sigy * ry; } while (ty >= bry || ty <= bly);
+                    if (!(rx * rx + ry * ry > 9.0)) break;
+                }
+                double tvx = vtx * rng.next();
+                double tvy = vty * rng.next();
+                const double tvz = vtz * rng.next() + vdz;
+                tvx = tvx - gamma * alpha_x / beta_x * (tx - x0);
+                tvy = tvy - gamma * alpha_y / beta_y * (ty - y0);
+                if (ty >= 0.0 && ty < (double)ny && tz >= 0.0 && tz < (double)nz) {
+                    if (npt < npmax) put(tx, ty, tz, tvx, tvy, tvz);
+                    else return -1;
+                    if (bp.quiet) {
+                        if (npt < npmax) put(x2 - tx, y2 - ty, tz, -tvx, -tvy, tvz);
+                        else return -1;
+                    }
+                }
+                l = l + 1;
+                if (bp.quiet) l = l + 1;
+            }
+            k = k + 1;
+        }
+        j = j + 1;
+    }
+    return npp;
+}
+
 // PGPOST32L (part3d_lib77.f:554-628), int64 fixed-point accumulation
 __global__ void k_beam_deposit(int64_t np, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q,
@@ -380,6 +464,46 @@ int qp_beam_create(qp_ctx *c, const qp_beam_params *p, double dt, uint64_t seed,
     double qm = 0.0;
     int64_t n = empty ? 0 : load_beam(c, *p, seed, h, npmax, &qm);
     int rc = n < 0 ? (set_error("qp_beam_create: npmax %lld too small", (long long)npmax),
+                      QP_ERR_ARG)
+                   : beam_alloc(b, npmax);
+    if (!rc) rc = beam_set(b, h, n);
+    if (rc) { qp_beam_destroy(b); return rc; }
+    *out = b;
+    return QP_OK;
+}
+
+int qp_beam_create_twiss(qp_ctx *c, const qp_beam_twiss_params *p, double dt, uint64_t seed,
+                         qp_beam **out)
+{
+    QP_REQUIRE(c && p && out, QP_ERR_ARG, "qp_beam_create_twiss: null argument");
+    QP_REQUIRE(p->q != 0.0 && p->m != 0.0 && p->gamma > 0.0 && p->beta[0] > 0.0 &&
+                   p->beta[1] > 0.0 && p->emittance[0] > 0.0 && p->emittance[1] > 0.0 &&
+                   p->sigma_z > 0.0 && p->np[0] > 0 && p->np[1] > 0 && p->np[2] > 0,
+               QP_ERR_ARG, "qp_beam_create_twiss: bad beam parameters");
+    int64_t want = (int64_t)p->np[0] * p->np[1] * p->np[2];
+    int64_t npmax = p->npmax > 0 ? p->npmax : want + 2;
+    QP_REQUIRE(npmax < ((int64_t)1 << 31), QP_ERR_UNSUPPORTED,
+               "qp_beam_create_twiss: npmax too large");
+    // the beam record keeps the profile-0 parameter block for push and diagnostics
+    qp_beam_params bp{};
+    for (int d = 0; d < 3; ++d) { bp.center[d] = p->center[d]; bp.np[d] = p->np[d]; }
+    bp.sigma[0] = std::sqrt(p->beta[0] * p->emittance[0] / p->gamma);
+    bp.sigma[1] = std::sqrt(p->beta[1] * p->emittance[1] / p->gamma);
+    bp.sigma[2] = p->sigma_z;
+    bp.sigma_v[2] = p->sigma_vz;
+    bp.gamma = p->gamma; bp.peak = p->peak; bp.q = p->q; bp.m = p->m;
+    bp.quiet = p->quiet; bp.evolution = p->evolution; bp.npmax = p->npmax;
+    qp_beam *b = new qp_beam();
+    b->ctx = c;
+    b->p = bp;
+    b->qbm = p->q / p->m;
+    b->dt = dt;
+    b->npmax = npmax;
+    std::vector<double> h[7];
+    for (auto &v : h) v.resize((size_t)npmax);
+    double qm = 0.0;
+    int64_t n = load_beam_twiss(c, *p, seed, h, npmax, &qm);
+    int rc = n < 0 ? (set_error("qp_beam_create_twiss: npmax %lld too small", (long long)npmax),
                       QP_ERR_ARG)
                    : beam_alloc(b, npmax);
     if (!rc) rc = beam_set(b, h, n);

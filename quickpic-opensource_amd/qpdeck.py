@@ -66,6 +66,25 @@ class BeamCfg:
 
 
 @dataclass
+class TwissCfg:
+    """Beam profile 2 (init_fdist3d_002, fdist3d_class.f03:652-745)."""
+    np_: tuple
+    center: tuple
+    alpha: tuple
+    beta: tuple
+    emittance: tuple
+    sigma_z: float
+    sigma_vz: float
+    gamma: float
+    peak: float
+    q: float = -1.0
+    m: float = 1.0
+    quiet: bool = True
+    evolution: bool = True
+    npmax: int = 0
+
+
+@dataclass
 class SpeciesCfg:
     ppc: tuple
     density: float = 1.0
@@ -109,8 +128,18 @@ def config(deck: dict) -> DeckCfg:
     for i in range(int(s["nbeams"])):
         b = deck["beam"][i]
         prof = int(b["profile"])
+        if prof == 2:
+            cfg.beams.append(TwissCfg(
+                np_=tuple(int(x) for x in b["np"]), center=_f3(b["center"]),
+                alpha=tuple(float(x) for x in b["alpha"]), beta=tuple(float(x) for x in b["beta"]),
+                emittance=tuple(float(x) for x in b["emittance"]),
+                sigma_z=float(b["sigmaz"]), sigma_vz=float(b["sigma_vz"]),
+                gamma=float(b["gamma"]), peak=float(b["peak_density"]), q=float(b["q"]),
+                m=float(b.get("m", 1.0)), quiet=bool(b.get("quiet_start", True)),
+                evolution=bool(b.get("evolution", True)), npmax=int(b.get("npmax", 0))))
+            continue
         if prof != 0:
-            raise ValueError(f"Invalid beam profile number: {prof} (this build supports 0)")
+            raise ValueError(f"Invalid beam profile number: {prof} (this build supports 0, 2)")
         cfg.beams.append(BeamCfg(
             np_=tuple(int(x) for x in b["np"]), center=_f3(b["center"]), sigma=_f3(b["sigma"]),
             sigma_v=_f3(b["sigma_v"]), gamma=float(b["gamma"]),
@@ -144,6 +173,13 @@ class Run:
                         for s in cfg.species]
         self.beams = []
         for i, b in enumerate(cfg.beams):
+            if isinstance(b, TwissCfg):
+                self.beams.append(qps.TwissBeam(
+                    self.ctx, center=b.center, alpha=b.alpha, beta=b.beta, emittance=b.emittance,
+                    sigma_z=b.sigma_z, sigma_vz=b.sigma_vz, np_=b.np_, gamma=b.gamma,
+                    peak=b.peak, q=b.q, m=b.m, quiet=b.quiet, evolution=b.evolution, dt=cfg.dt,
+                    seed=seed + i, npmax=b.npmax))
+                continue
             self.beams.append(qps.Beam(self.ctx, center=b.center, sigma=b.sigma,
                                        sigma_v=b.sigma_v, np_=b.np_, gamma=b.gamma,
                                        peak=b.peak, q=b.q, m=b.m, quiet=b.quiet,

@@ -63,6 +63,12 @@ class BeamParams(C.Structure):
                 ("np", _I * 3), ("quiet", _I), ("evolution", _I), ("npmax", _I64)]
 
 
+class BeamTwissParams(C.Structure):
+    _fields_ = [("center", _D * 3), ("alpha", _D * 2), ("beta", _D * 2), ("emittance", _D * 2),
+                ("sigma_z", _D), ("sigma_vz", _D), ("gamma", _D), ("peak", _D), ("q", _D),
+                ("m", _D), ("np", _I * 3), ("quiet", _I), ("evolution", _I), ("npmax", _I64)]
+
+
 def lib():
     """Loads libqpslice.so; fails loudly when it has not been built."""
     global _lib
@@ -462,6 +468,31 @@ class Beam:
             self.close()
         except Exception:
             pass
+
+
+class TwissBeam(Beam):
+    """Beam profile 2 (fdist3d_002): Twiss transverse phase space, Gaussian
+    in z.  Lengths in c/wp as in the input deck."""
+
+    def __init__(self, ctx: Ctx, center=(0.0, 0.0, 2.0), alpha=(0.0, 0.0), beta=(1.0, 1.0),
+                 emittance=(1e-4, 1e-4), sigma_z=0.5, sigma_vz=0.0, np_=(64, 64, 256),
+                 gamma=2.0e4, peak=4.0, q=-1.0, m=1.0, quiet=True, evolution=True, dt=10.0,
+                 seed=0, npmax=0):
+        self.ctx = ctx
+        p = BeamTwissParams()
+        for i in range(3):
+            p.center[i], p.np[i] = center[i], np_[i]
+        for i in range(2):
+            p.alpha[i], p.beta[i], p.emittance[i] = alpha[i], beta[i], emittance[i]
+        p.sigma_z, p.sigma_vz = sigma_z, sigma_vz
+        p.gamma, p.peak, p.q, p.m = gamma, peak, q, m
+        p.quiet = 1 if quiet else 0
+        p.evolution = 1 if evolution else 0
+        p.npmax = npmax
+        self.h = _P()
+        _call(lib().qp_beam_create_twiss, ctx.h, C.byref(p), _D(dt), C.c_uint64(seed),
+              C.byref(self.h))
+        ctx._adopt(self)
 
 
 class Sim:

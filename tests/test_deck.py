@@ -38,6 +38,18 @@ def test_unsupported_profile_is_rejected():
         qpdeck.config(deck)
 
 
+def test_twiss_beam_is_read():
+    """A profile-2 beam (init_fdist3d_002 keys) becomes a Twiss beam."""
+    deck = qpdeck.load(DECK)
+    deck["beam"][0].update({"profile": 2, "alpha": [0.7, -0.4], "beta": [10.0, 8.0],
+                            "emittance": [3.9, 5.0], "sigmaz": 0.6, "sigma_vz": 3.0})
+    cfg = qpdeck.config(deck)
+    b = cfg.beams[0]
+    assert isinstance(b, qpdeck.TwissCfg)
+    assert b.alpha == (0.7, -0.4) and b.beta == (10.0, 8.0) and b.emittance == (3.9, 5.0)
+    assert b.sigma_z == 0.6 and b.sigma_vz == 3.0 and b.np_ == (16, 16, 32)
+
+
 @pytest.mark.gpu
 def test_deck_run_matches_oracle():
     from oracle import qpo

@@ -290,6 +290,23 @@ def test_beam_loader_and_deposit(ctx):
     assert abs(g - gam) <= 1e-12 * gam
 
 
+def test_twiss_beam_loader(ctx):
+    """Profile 2 (fdist3d_002 / PRVDIST32_TWISS): the library's host loader
+    and the oracle draw the same ranorm stream; particles bit-exact."""
+    banner("twiss beam loader")
+    grid = qpo.Grid(IND, IND, 6)
+    spec = qpo.TwissSpec(center=(0.3, -0.2, 5.0), alpha=(0.7, -0.4), beta=(10.0, 8.0),
+                         emittance=(3.9, 5.0), sigma_z=0.6, sigma_vz=3.0, np_=(16, 16, 32),
+                         gamma=100.0, peak=2.5)
+    want, qm, _ = qpo.beam_init_twiss(grid, spec)
+    b = qps.TwissBeam(ctx, center=spec.center, alpha=spec.alpha, beta=spec.beta,
+                      emittance=spec.emittance, sigma_z=spec.sigma_z, sigma_vz=spec.sigma_vz,
+                      np_=spec.np_, gamma=spec.gamma, peak=spec.peak, dt=10.0)
+    got = b.download()
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
 def test_beam_push(ctx):
     banner("beam push")
     grid = qpo.Grid(IND, IND, 6)
