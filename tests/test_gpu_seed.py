@@ -138,3 +138,38 @@ def test_features_are_block_means_of_the_named_planes():
         assert rel(xn[ch], raw[ch] / r) < 1e-5
         if g == 1:
             assert abs(sd / r - 1) < 1e-5
+
+
+def test_nonzero_correction_enters_the_seed():
+    """A known correction (constant 0.5 on the network's grid) reaches the
+    corrector: epw after seed() = 2 epw(j-1) - epw(j-2) + the kind-1 forward
+    transform of the correction on the nodes (constant 0.5 where the coarse
+    grid covers them, zero on the guard nodes)."""
+    banner("nonzero correction")
+    ctx, sim = make()
+    ctx.tune(qps.TUNE_SEED, 2)
+    f = 2
+    with torch.cuda.stream(qpseed.ctx_stream(ctx)):
+        fs = qpseed.SeedFeatures(ctx, sim, f)
+        for j in range(1, 5):
+            sim.prepare(j)
+            fs.seed()
+            sim.finish(j, iter_max=20, tol=TOL)
+            fs.record()
+        sim.prepare(5)
+        ctx.sync()
+        e1 = sim.field(qps.F_EPW).download(1)
+        e2 = fs.hk.download(1)
+        corr = torch.full((2, fs.H * fs.W), 0.5, device="cuda")
+        fs.seed(corr, 1.0)
+        ctx.sync()
+        got = sim.field(qps.F_EPW).download(1)
+    ny, nx = ctx.ny, ctx.nx
+    c = np.zeros((2, ny + 1, nx + 1))
+    c[:, : f * fs.H, : f * fs.W] = 0.5
+    t = qps.Field2d(ctx, 2, state=0)
+    t.upload(c, space=0)
+    t.fftrk(1)
+    want = 2.0 * e1 - e2 + t.download(1)
+    assert rel(got, want) < 1e-5
+    assert rel(got, 2.0 * e1 - e2) > 1e-3   # the correction is not negligible here
