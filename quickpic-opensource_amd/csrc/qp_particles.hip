@@ -646,6 +646,123 @@ k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, cons
         emit_nodes(nacc + c * 289, out, c, g.ldr, halo + c * halo_cs, tile, tx, ty);
 }
 
+// FP32 deposit with the particle chunks copied from memory into LDS by the
+// LDS-DMA loads (global_load_lds_dword: each lane's dword lands at the wave's
+// LDS base + 4 lane) into two buffers: the next chunk is in flight while the
+// current one is computed, and the copy uses no registers.  Chunks hold CH
+// particles; LDS index of particle p of a chunk: p + p/64 (one pad per
+// wave block).  Cells, lanes, sums and their order are those of k_amjdp.
+__device__ __forceinline__ int lpad64(int p) { return p + (p >> 6); }
+
+template <int RG, int CH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_amjdp_dma(TileGeom g, Soa<float> s, const int *__restrict__ off, const float *fxy,
+            const float *bxy, const float *psit, size_t cs, DepOut<float> out, float *halo,
+            size_t halo_cs, float qbm, float dt, float dex, const int *skip, const float *psi)
+{
+    using R = float;
+    if (skip && *skip) return;
+    constexpr int T = TILE / RG, NB = CH + CH / 64, K = CH / TCELLS;
+    extern __shared__ unsigned char smem_raw[];
+    R *base = reinterpret_cast<R *>(smem_raw);          // [2 buffers][7 arrays][NB]
+    R *nacc = base + 2 * 7 * NB;                         // 8 x 289
+    const int tid = threadIdx.x, tile = blockIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
+    for (int n = tid; n < 8 * 289; n += TCELLS) nacc[n] = R(0);
+    const int cl = tid / T, sub = tid % T;
+    const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
+    const R *src[7] = {s.x, s.y, s.px, s.py, s.p6, s.p7, s.q};
+    // copy chunk [b0, b0 + nn) into buffer bf: wave w takes blocks k*256 + 64 w
+    auto issue = [&](int b0, int nn, int bf) {
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int blk = k * TCELLS + wave * 64;
+                if (blk + lane < nn)
+                    __builtin_amdgcn_global_load_lds(
+                        src[a] + b0 + blk + lane,
+                        (__attribute__((address_space(3))) void *)(base + (bf * 7 + a) * NB +
+                                                                   lpad64(blk)),
+                        4, 0, 0);
+            }
+        }
+    };
+    int pc0 = -1, pn = 0, cur = 0;
+    for (int row = 0; row < TILE; row += RG) {
+        const int lrow = row + cl / TILE, lcol = cl % TILE;
+        const int key = kb + lrow * TILE + lcol;
+        const int i0 = off[key], i1 = off[key + 1];
+        const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
+        const int G2 = row + RG < TILE ? off[kb + (row + 2 * RG) * TILE] : G1;
+        const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
+        Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        prep_fields(F);
+        const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
+        const R fx = (R)cx, fy = (R)cy;
+        R acc[8][4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
+        for (int c0 = G0; c0 < G1; c0 += CH) {
+            const int n = min(CH, G1 - c0);
+            if (pc0 != c0 || pn != n) issue(c0, n, cur);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            // the next chunk: the rest of this group, or the start of the next
+            const int nc0 = c0 + n;
+            const int nn = nc0 < G1 ? min(CH, G1 - nc0) : min(CH, G2 - nc0);
+            if (nn > 0) {
+                issue(nc0, nn, cur ^ 1);
+                pc0 = nc0;
+                pn = nn;
+            } else {
+                pc0 = -1;
+            }
+            R *sx = base + (cur * 7 + 0) * NB, *sy = sx + NB, *spx = sy + NB, *spy = spx + NB;
+            R *sp6 = spy + NB, *sp7 = sp6 + NB, *sq = sp7 + NB;
+            int a = i0 + sub;
+            if (a < c0) a += ((c0 - a + T - 1) / T) * T;
+            const int b = min(i1, c0 + n);
+            for (int i = a; i < b; i += T) {
+                const int l = lpad64(i - c0);
+                R p6 = sp6[l], p7 = sp7[l];
+                if (psi) {
+                    extpsi_one(FP, fx, fy, sx[l], sy[l], spx[l], spy[l], qbm, dex, p6, p7);
+                    sp7[l] = p7;
+                }
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l], qtmh,
+                                   qbm, dti, dex, idex, acc);
+            }
+            __syncthreads();
+            for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad64(p)];
+            if (psi)
+                for (int p = tid; p < n; p += TCELLS) s.p7[c0 + p] = sp7[lpad64(p)];
+            cur ^= 1;
+        }
+#pragma unroll
+        for (int o = T / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[c][k] += __shfl_xor(acc[c][k], o);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            __syncthreads();
+            if (sub == 0) {
+                const int node = (lrow + (k >> 1)) * 17 + lcol + (k & 1);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) nacc[c * 289 + node] += acc[c][k];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        emit_nodes(nacc + c * 289, out, c, g.ldr, halo + c * halo_cs, tile, tx, ty);
+}
+
 // Orders LDS accesses among the lanes of one wave (no workgroup barrier).
 __device__ __forceinline__ void wave_sync()
 {
@@ -1138,7 +1255,9 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
 struct DepShape {
     int rg, ch, wpe;
     int wave;          // 1: k_amjdp_w (independent waves, staging chunk ch per wave);
-                       // 2: k_amjdp_pf (next chunk prefetched into registers)
+                       // 2: k_amjdp_pf (next chunk prefetched into registers);
+                       // 3: k_amjdp_dma (FP32: chunks copied into LDS by LDS-DMA,
+                       //    double-buffered)
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
@@ -1147,7 +1266,7 @@ template <> struct DepTab<float> {
     static constexpr DepShape j[] = {{8, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
-                                     {8, 512, 1, 2}};
+                                     {8, 512, 1, 2},   {8, 512, 1, 3}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
@@ -1155,7 +1274,7 @@ template <> struct DepTab<double> {
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
                                      {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
-                                     {8, 256, 1, 2}};
+                                     {8, 256, 1, 2},   {8, 256, 1, 0}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1310,6 +1429,10 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     } else if constexpr (d.wave == 2) {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         kern = k_amjdp_pf<R, d.rg, d.ch>;
+    } else if constexpr (d.wave == 3) {
+        static_assert(std::is_same<R, float>::value, "LDS-DMA staging is FP32 only");
+        lds = (2 * 7 * (size_t)(d.ch + d.ch / 64) + 8 * 289) * sizeof(R);
+        kern = k_amjdp_dma<d.rg, d.ch>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
@@ -1345,6 +1468,7 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 7: rc = amjdp_v<R, 7>(sp, ef, bf, psit, dex, out, psi); break;
     case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out, psi); break;
     case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
+    case 10: rc = amjdp_v<R, 10>(sp, ef, bf, psit, dex, out, psi); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;
