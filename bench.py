@@ -142,8 +142,8 @@ def run_gpu(args, precision, beam_part=None):
             sweep[f"seed[{v}] passes"] = float(np.mean(its[2:]))
             j0 += 10
         ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
-        # transforms: transpose kernels / column-pair y pass
-        for v in (0, 1):
+        # transforms: transpose kernels / column-pair y pass / LDS paddings
+        for v in range(ctx.tune_get(qps.TUNE_FFT)[1]):
             ctx.tune(qps.TUNE_FFT, v)
             ctx.sync()
             t0 = time.perf_counter()

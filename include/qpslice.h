@@ -113,7 +113,9 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * (tol <= 0) use the reference's seed whatever the knob.
  * QP_TUNE_FFT = 1 folds the transpose between the x and y passes of every
  * transform into the y pass (column-pair reads / writes) instead of a
- * separate LDS-tiled transpose (default 0); results are identical.
+ * separate LDS-tiled transpose (default 0); values 2, 3, 4 keep the
+ * transpose and pad the transforms' LDS by one point per 16, 32 or 4 points
+ * (default per 8); results are identical.
  * value -1 restores a knob's default. */
 enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_SEED = 3,
        QP_TUNE_FFT = 4, QP_TUNE_COUNT };

@@ -69,9 +69,10 @@ __device__ __forceinline__ void dft8(cplx<R> *v)
     v[1] = c0; v[3] = c1; v[5] = c2; v[7] = c3;
 }
 
-__device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
+// LDS index of FFT point i: one pad point after every 2^PS points
+template <int PS> __device__ __forceinline__ int padk(int i) { return i + (i >> PS); }
 
-template <int L> struct FftShape {
+template <int L, int PS = 3> struct FftShape {
     static constexpr int M = 1 << L;                // complex points
     static constexpr int N = M / 2;                 // transform length n
     static constexpr int TF = M / 8;                // threads per transform
@@ -79,7 +80,7 @@ template <int L> struct FftShape {
     static constexpr int FB = BS / TF;              // transforms per block
     static constexpr int NR8 = L / 3;               // radix-8 passes
     static constexpr int REM = L % 3;               // closing radix-2 (1) / radix-4 (2)
-    static constexpr int LDSM = M + M / 8;          // padded LDS points per transform
+    static constexpr int LDSM = M + (M >> PS) + 1;  // padded LDS points per transform
 };
 
 template <typename R> struct CompSet {
@@ -97,12 +98,12 @@ template <typename R> struct CompSet {
 // Either way a lane moves both rows of its pair with one 2-element access,
 // and block b takes pair (b mod 8) * (gridDim.x / 8) + b / 8, so the blocks
 // on one XCD (dispatched round-robin by block id) share cache lines.
-template <typename R, int L, int COL>
+template <typename R, int L, int COL, int PS>
 __global__ void __launch_bounds__(FftShape<L>::BS)
 k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
            const cplx<R> *__restrict__ W, int guard_row, const int *skip, ResJob rj)
 {
-    using S = FftShape<L>;
+    using S = FftShape<L, PS>;
     if (skip && *skip) return;
     if (blockIdx.x == 0 && blockIdx.y == 0) run_res_job(rj);
     constexpr int M = S::M, n = S::N, TF = S::TF;
@@ -221,7 +222,7 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     }
     // first pass output (Ns = 1): index 8 t + r
 #pragma unroll
-    for (int r = 0; r < 8; ++r) buf[pad8(8 * t + r)] = v[r];
+    for (int r = 0; r < 8; ++r) buf[padk<PS>(8 * t + r)] = v[r];
 
     // middle radix-8 passes
 #pragma unroll
@@ -235,7 +236,7 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
             const cplx<R> w[8] = {w1, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4)};
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                v[r] = buf[pad8(t + r * TF)];
+                v[r] = buf[padk<PS>(t + r * TF)];
                 if (r) v[r] = cmul(v[r], w[r]);
             }
         }
@@ -249,7 +250,7 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
         __syncthreads();
         const int base = (t / Ns) * Ns * 8 + (t & (Ns - 1));
 #pragma unroll
-        for (int r = 0; r < 8; ++r) buf[pad8(base + r * Ns)] = v[r];
+        for (int r = 0; r < 8; ++r) buf[padk<PS>(base + r * Ns)] = v[r];
     }
     __syncthreads();
     if (S::REM == 2) {
@@ -258,10 +259,10 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
         for (int s = 0; s < 2; ++s) {
             const int j = t + s * TF;
             const cplx<R> w1 = twr[2 * s], w2 = twr[2 * s + 1];
-            cplx<R> a0 = buf[pad8(j)];
-            cplx<R> a1 = cmul(buf[pad8(j + M / 4)], w1);
-            cplx<R> a2 = cmul(buf[pad8(j + M / 2)], w2);
-            cplx<R> a3 = cmul(buf[pad8(j + 3 * M / 4)], cmul(w1, w2));
+            cplx<R> a0 = buf[padk<PS>(j)];
+            cplx<R> a1 = cmul(buf[padk<PS>(j + M / 4)], w1);
+            cplx<R> a2 = cmul(buf[padk<PS>(j + M / 2)], w2);
+            cplx<R> a3 = cmul(buf[padk<PS>(j + 3 * M / 4)], cmul(w1, w2));
             dft4(a0, a1, a2, a3);
             emit(j, a0);
             emit(j + M / 4, a1);
@@ -272,8 +273,8 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int j = t + s * TF;
-            cplx<R> a0 = buf[pad8(j)];
-            cplx<R> a1 = cmul(buf[pad8(j + M / 2)], twr[s]);
+            cplx<R> a0 = buf[padk<PS>(j)];
+            cplx<R> a1 = cmul(buf[padk<PS>(j + M / 2)], twr[s]);
             emit(j, cadd(a0, a1));
             emit(j + M / 2, csub(a0, a1));
         }
@@ -321,16 +322,16 @@ void kind_types(int kind, int dim, int *tx, int *ty)
 
 static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
-template <typename R, int L, int COL>
+template <typename R, int L, int COL, int PS>
 static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int ld_src,
                        int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
-    using S = FftShape<L>;
+    using S = FftShape<L, PS>;
     const size_t lds = (size_t)S::FB * S::LDSM * sizeof(cplx<R>);
     QP_REQUIRE(lds <= c->max_lds, QP_ERR_UNSUPPORTED,
                "transform of length %d needs %zu B of LDS (limit %zu)", S::N, lds, c->max_lds);
     QP_REQUIRE(!COL || guard_row < 0, QP_ERR_ARG, "column transforms take no guard row");
-    auto kern = k_fft_rows<R, L, COL>;
+    auto kern = k_fft_rows<R, L, COL, PS>;
     if (lds > 65536)
         QP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
@@ -349,20 +350,20 @@ static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, in
     return QP_OK;
 }
 
-template <typename R, int COL>
+template <typename R, int COL, int PS = 3>
 static int rowfft(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int n, int ld_src,
                   int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
     switch (ilog2(2 * n)) {
-    case 5: return rows_launch<R, 5, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 6: return rows_launch<R, 6, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 7: return rows_launch<R, 7, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 8: return rows_launch<R, 8, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 9: return rows_launch<R, 9, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 10: return rows_launch<R, 10, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 11: return rows_launch<R, 11, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 12: return rows_launch<R, 12, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
-    case 13: return rows_launch<R, 13, COL>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 5: return rows_launch<R, 5, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 6: return rows_launch<R, 6, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 7: return rows_launch<R, 7, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 8: return rows_launch<R, 8, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 9: return rows_launch<R, 9, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 10: return rows_launch<R, 10, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 11: return rows_launch<R, 11, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 12: return rows_launch<R, 12, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
+    case 13: return rows_launch<R, 13, COL, PS>(c, cs, ncomp, nrows, ld_src, ld_dst, types, scale, W, guard_row);
     }
     set_error("transform length %d unsupported", n);
     return QP_ERR_UNSUPPORTED;
@@ -380,6 +381,20 @@ static int transpose(qp_ctx *c, const CompSet<R> &cs, int ncomp, int ld_src, int
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
+}
+
+// the row transforms of the transpose path; QP_TUNE_FFT values 2-4 select
+// another LDS padding (one pad point per 16, 32 or 4 points instead of 8)
+template <typename R>
+static int rowfft0(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int n, int ld_src,
+                   int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
+{
+    switch (c->tune[QP_TUNE_FFT]) {
+    case 2: return rowfft<R, 0, 4>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
+    case 3: return rowfft<R, 0, 5>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
+    case 4: return rowfft<R, 0, 2>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
+    default: return rowfft<R, 0, 3>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
+    }
 }
 
 // components of a batch of fields: plane pointers and type masks
@@ -424,14 +439,14 @@ static int r2k_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
         y.dst[i] = ks[i];
     }
     const double ani = 1.0 / ((double)c->nx * (double)c->ny);
-    int rc = rowfft<R, 0>(c, a, nc, c->ny + 1, c->nx, c->ldr, c->ldr, mx, ani,
+    int rc = rowfft0<R>(c, a, nc, c->ny + 1, c->nx, c->ldr, c->ldr, mx, ani,
                           c->twx.as<cplx<R>>(), -1);
     if (rc) return rc;
     if (c->tune[QP_TUNE_FFT] == 1)   // y pass straight from the column pairs of the x pass
         return rowfft<R, 1>(c, b, nc, c->nx + 1, c->ny, c->ldr, c->ldk, my, 1.0,
                             c->twy.as<cplx<R>>(), -1);
     rc = transpose<R>(c, b, nc, c->ldr, c->ldk, c->ny + 1, c->nx + 1);
-    if (!rc) rc = rowfft<R, 0>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
+    if (!rc) rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
                                c->twy.as<cplx<R>>(), -1);
     return rc;
 }
@@ -460,12 +475,12 @@ static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
         rc = rowfft<R, 2>(c, yc, nc, c->nx + 1, c->ny, c->ldk, c->ldr, my, 2.0,
                           c->twy.as<cplx<R>>(), -1);
     } else {
-        rc = rowfft<R, 0>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,
+        rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,
                           c->twy.as<cplx<R>>(), -1);
         if (!rc) rc = transpose<R>(c, b, nc, c->ldk, c->ldr, c->nx + 1, c->ny + 1);
     }
     // rows 0..ny-1; row ny receives row 0 (guard copy, PPNCGUARD2L nvp = 1)
-    if (!rc) rc = rowfft<R, 0>(c, a, nc, c->ny, c->nx, c->ldr, c->ldr, mx, 2.0,
+    if (!rc) rc = rowfft0<R>(c, a, nc, c->ny, c->nx, c->ldr, c->ldr, mx, 2.0,
                                c->twx.as<cplx<R>>(), c->ny);
     return rc;
 }

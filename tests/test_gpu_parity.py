@@ -80,14 +80,15 @@ def test_transforms(ctx, kind, dim):
 
 @pytest.mark.parametrize("kind,dim", [(1, 1), (2, 3), (3, 3)])
 def test_transforms_column_mode_identical(ctx, kind, dim):
-    """The y pass reading / writing column pairs (QP_TUNE_FFT = 1) gives the
-    same bits as the transpose kernel path, and so the oracle's result."""
+    """Every transform variant (QP_TUNE_FFT: the column-pair y pass, the LDS
+    paddings) gives the same bits as the default path, and so the oracle's
+    result."""
     banner(f"transforms column mode kind={kind} dim={dim}")
     rng = np.random.default_rng(100 + kind * 10 + dim)
     r = rng.standard_normal((dim, NY + 1, NX + 1))
     k = rng.standard_normal((dim, NX + 1, NY + 1))
     out = []
-    for mode in (0, 1):
+    for mode in range(ctx.tune_get(qps.TUNE_FFT)[1]):
         ctx.tune(qps.TUNE_FFT, mode)
         f = qps.Field2d(ctx, dim)
         f.upload(r, space=0)
@@ -97,7 +98,8 @@ def test_transforms_column_mode_identical(ctx, kind, dim):
         f.fftkr(kind)
         out.append((a, f.download(0)))
     ctx.tune(qps.TUNE_FFT, -1)
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for m in range(1, len(out)):
+        assert np.array_equal(out[0][0], out[m][0]) and np.array_equal(out[0][1], out[m][1]), m
     assert rel(out[1][0], qpo.r2k(kind, r)) < (1e-12 if ctx.precision == qps.FP64 else 1e-5)
 
 
