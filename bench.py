@@ -142,17 +142,32 @@ def run_gpu(args, precision, beam_part=None):
             sweep[f"seed[{v}] passes"] = float(np.mean(its[2:]))
             j0 += 10
         ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
-        # transforms: transpose kernels / column-pair y pass / LDS paddings
+        # kernel time per slice of the classes a knob changes (HIP events on
+        # the library's stream), so the pass count of the slices drops out
+        def class_ms(knob, v, names, nsl=5):
+            nonlocal j0
+            ctx.tune(knob, v)
+            ctx.profile(True)
+            passes = 0
+            for j in range(j0, j0 + nsl):
+                passes += sim.slice(j, iter_max=args.iter_max, tol=args.tol)[0]
+            t = sum(ctx.profile_get(qps.PROF_NAMES.index(n))[0] for n in names)
+            work = ctx.profile_get(qps.PROF_NAMES.index(names[0]))[2]
+            ctx.profile(False)
+            j0 += nsl
+            return round(t / nsl, 4), passes / nsl, t / max(work, 1.0) * 1e9
+        # transforms: transpose kernels / column-pair y pass / LDS paddings;
+        # ms per GB of row-transform work does not depend on the pass count
         for v in range(ctx.tune_get(qps.TUNE_FFT)[1]):
-            ctx.tune(qps.TUNE_FFT, v)
-            ctx.sync()
-            t0 = time.perf_counter()
-            for j in range(j0, j0 + 5):
-                sim.slice(j, iter_max=args.iter_max, tol=args.tol)
-            ctx.sync()
-            sweep[f"fft[{v}] ms/slice"] = round((time.perf_counter() - t0) / 5 * 1e3, 4)
-            j0 += 5
+            t, ps, per_gb = class_ms(qps.TUNE_FFT, v, ("fft", "transpose"))
+            sweep[f"fft[{v}] ms/slice"] = t
+            sweep[f"fft[{v}] passes"] = ps
+            sweep[f"fft[{v}] ms/GB"] = round(per_gb, 4)
         ctx.tune(qps.TUNE_FFT, -1)
+        # the push's re-sort: count then push into place / push then scatter
+        for v in range(ctx.tune_get(qps.TUNE_PUSH)[1]):
+            sweep[f"push[{v}] ms/slice"] = class_ms(qps.TUNE_PUSH, v, ("push", "sort"))[0]
+        ctx.tune(qps.TUNE_PUSH, -1)
         log(f"[bench] sweep (ms per launch): {sweep}")
     # every kernel class: time, launches and algorithmic bytes (operands read
     # or written once) counted by the library at each launch

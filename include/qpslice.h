@@ -114,11 +114,16 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * QP_TUNE_FFT = 1 folds the transpose between the x and y passes of every
  * transform into the y pass (column-pair reads / writes) instead of a
  * separate LDS-tiled transpose (default 0); values 2, 3, 4 keep the
- * transpose and pad the transforms' LDS by one point per 16, 32 or 4 points
- * (default per 8); results are identical.
+ * transpose and pad the transforms' LDS by one point per 8, 32 or 4 points
+ * (default per 16 in FP32, per 8 in FP64); results are identical.
+ * QP_TUNE_PUSH selects how the plasma push re-sorts the particles: 0 (default)
+ * counts the destination bins in a first pass that stores nothing per
+ * particle, then pushes again and stores every particle at its sorted place;
+ * 1 pushes in place, keeping each particle's bin and rank, then scatters.
+ * Both give the same arrays, bit for bit.
  * value -1 restores a knob's default. */
 enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_SEED = 3,
-       QP_TUNE_FFT = 4, QP_TUNE_COUNT };
+       QP_TUNE_FFT = 4, QP_TUNE_PUSH = 5, QP_TUNE_COUNT };
 int qp_ctx_tune(qp_ctx *ctx, int knob, int value);
 /* current value and the number of values of a knob */
 int qp_ctx_tune_get(const qp_ctx *ctx, int knob, int *value, int *nvalues);

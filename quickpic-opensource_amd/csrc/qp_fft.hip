@@ -350,7 +350,7 @@ static int rows_launch(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, in
     return QP_OK;
 }
 
-template <typename R, int COL, int PS = 3>
+template <typename R, int COL, int PS>
 static int rowfft(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int n, int ld_src,
                   int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
@@ -383,17 +383,23 @@ static int transpose(qp_ctx *c, const CompSet<R> &cs, int ncomp, int ld_src, int
     return QP_OK;
 }
 
+// LDS padding of the transforms by default: one pad point per 16 points in
+// FP32 (a third fewer LDS bank conflicts than per 8 at 2048 points, measured
+// on MI355X), per 8 in FP64
+template <typename R> constexpr int pad_shift() { return sizeof(R) == 4 ? 4 : 3; }
+
 // the row transforms of the transpose path; QP_TUNE_FFT values 2-4 select
-// another LDS padding (one pad point per 16, 32 or 4 points instead of 8)
+// another LDS padding (one pad point per 8, 32 or 4 points)
 template <typename R>
 static int rowfft0(qp_ctx *c, const CompSet<R> &cs, int ncomp, int nrows, int n, int ld_src,
                    int ld_dst, int types, double scale, const cplx<R> *W, int guard_row)
 {
+    constexpr int PD = pad_shift<R>();
     switch (c->tune[QP_TUNE_FFT]) {
-    case 2: return rowfft<R, 0, 4>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
+    case 2: return rowfft<R, 0, 3>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
     case 3: return rowfft<R, 0, 5>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
     case 4: return rowfft<R, 0, 2>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
-    default: return rowfft<R, 0, 3>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
+    default: return rowfft<R, 0, PD>(c, cs, ncomp, nrows, n, ld_src, ld_dst, types, scale, W, guard_row);
     }
 }
 
@@ -443,7 +449,7 @@ static int r2k_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
                           c->twx.as<cplx<R>>(), -1);
     if (rc) return rc;
     if (c->tune[QP_TUNE_FFT] == 1)   // y pass straight from the column pairs of the x pass
-        return rowfft<R, 1>(c, b, nc, c->nx + 1, c->ny, c->ldr, c->ldk, my, 1.0,
+        return rowfft<R, 1, pad_shift<R>()>(c, b, nc, c->nx + 1, c->ny, c->ldr, c->ldk, my, 1.0,
                             c->twy.as<cplx<R>>(), -1);
     rc = transpose<R>(c, b, nc, c->ldr, c->ldk, c->ny + 1, c->nx + 1);
     if (!rc) rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
@@ -472,7 +478,7 @@ static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
     }
     int rc;
     if (c->tune[QP_TUNE_FFT] == 1) {   // y pass written as column pairs of rs
-        rc = rowfft<R, 2>(c, yc, nc, c->nx + 1, c->ny, c->ldk, c->ldr, my, 2.0,
+        rc = rowfft<R, 2, pad_shift<R>()>(c, yc, nc, c->nx + 1, c->ny, c->ldk, c->ldr, my, 2.0,
                           c->twy.as<cplx<R>>(), -1);
     } else {
         rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,

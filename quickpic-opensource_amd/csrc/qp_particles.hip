@@ -29,6 +29,7 @@ struct qp_species {
     qp::DevBuf key, slot;      // per particle: destination bin and rank (push), key and index (load)
     qp::DevBuf bins;           // per tile: counts, then starts, of the 9 x 256 destination bins
     qp::DevBuf count, off;     // per cell (ncells + 1)
+    qp::DevBuf off_new;        // the push's new cell offsets, swapped with off
     qp::DevBuf scan_tmp, err;
     size_t scan_bytes;
     qp_field2d *q, *qn, *cu, *dcu, *amu;
@@ -874,6 +875,102 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
     }
 }
 
+// The deposit of k_amjdp_w with no staging: each lane reads its particles
+// straight from memory, the record of its next particle loaded while the
+// current one is computed, and writes p6 (with psi also p7) back in place.
+// A wave's 32 cells (one row pair) hold one contiguous run of the sorted
+// arrays, so its loads fall on the few cache lines of that run; no barrier
+// orders the particle loop.  Lanes, cells and the order of the sums are those
+// of k_amjdp_w.
+template <typename R, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_amjdp_d(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
+          const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
+          const int *skip, const R *psi)
+{
+    if (skip && *skip) return;
+    constexpr int NR = 3 * 17;   // node rows of a row pair
+    extern __shared__ unsigned char smem_raw[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    R *rnode = reinterpret_cast<R *>(smem_raw);       // [row pair 8][comp 8][3][17]
+    const int tile = blockIdx.x, tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
+    for (int n = tid; n < 8 * 8 * NR; n += TCELLS) rnode[n] = R(0);
+    __syncthreads();
+    const int cl = lane >> 1, sub = lane & 1;
+    const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
+    for (int it = 0; it < 2; ++it) {
+        const int rp = wave + 4 * it;
+        const int lrow = 2 * rp + (cl >> 4), lcol = cl & 15;
+        const int key = kb + lrow * TILE + lcol;
+        const int i0 = off[key], i1 = off[key + 1];
+        const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
+        Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
+        prep_fields(F);
+        const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
+        const R fx = (R)cx, fy = (R)cy;
+        R acc[8][4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
+        R x = R(0), y = R(0), px = R(0), py = R(0), p6 = R(1), p7 = R(1), q = R(0);
+        auto load = [&](int i) {
+            x = s.x[i];
+            y = s.y[i];
+            px = s.px[i];
+            py = s.py[i];
+            p6 = s.p6[i];
+            p7 = s.p7[i];
+            q = s.q[i];
+        };
+        int i = i0 + sub;
+        if (i < i1) load(i);
+        for (; i < i1; i += 2) {
+            const R cx_ = x, cy_ = y, cpx = px, cpy = py, cq = q;
+            R c6 = p6, c7 = p7;
+            if (i + 2 < i1) load(i + 2);
+            if (psi) {
+                extpsi_one(FP, fx, fy, cx_, cy_, cpx, cpy, qbm, dex, c6, c7);
+                s.p7[i] = c7;
+            }
+            s.p6[i] = amjdp_one(F, fx, fy, cx_, cy_, cpx, cpy, c6, c7, cq, qtmh, qbm, dti, dex,
+                                idex, acc);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[c][k] += __shfl_xor(acc[c][k], 1);
+        R *rn = rnode + rp * 8 * NR;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            wave_sync();
+            if (sub == 0) {
+                const int node = ((cl >> 4) + (k >> 1)) * 17 + lcol + (k & 1);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) rn[c * NR + node] += acc[c][k];
+            }
+        }
+    }
+    __syncthreads();
+    for (int n = tid; n < 17 * 17; n += TCELLS) {
+        const int i = n % 17, j = n / 17;
+        const int rp = j >> 1;
+        const bool top = (j & 1) == 0 && j > 0;
+        for (int c = 0; c < 8; ++c) {
+            R v;
+            if (j == 16) v = rnode[(7 * 8 + c) * NR + 2 * 17 + i];
+            else {
+                v = rnode[(rp * 8 + c) * NR + (j & 1) * 17 + i];
+                if (top) v = rnode[((rp - 1) * 8 + c) * NR + 2 * 17 + i] + v;
+            }
+            if (i >= 1 && i < TILE && j >= 1 && j < TILE)
+                dep_store(out, c, (size_t)(ty * TILE + j) * g.ldr + tx * TILE + i, v);
+            else
+                halo[c * halo_cs + (size_t)tile * HALO + halo_index(i, j)] = v;
+        }
+    }
+}
+
 // PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
 // periodic wrap of the reorder (PPPORDER2LB :1936-1977).  One workgroup per
 // source tile walks the tile's particles (a contiguous run of the sorted
@@ -887,12 +984,22 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
 // run to run.  The histogram goes to cnt[tile][nb][cl].  A particle whose new
 // tile is not its old tile plus its direction (the condition PPPCHECK2L
 // reports) sets *err and is kept in its own tile.
+//
+// MODE 0 stores the pushed particle in place with its bin and rank (k_scatter
+// then moves it).  MODE 1 stores nothing per particle: only the histogram.
+// MODE 2 runs after MODE 1 and k_bin_starts (cnt then holds each bin's start
+// within its cell) and the scan of the new cell counts into off_new: it
+// repeats the push on the unchanged input, so bins and ranks are those of
+// MODE 1, and stores each particle at off_new[cell] + start + rank of the
+// alternate arrays o.  The two passes read the particle record twice and
+// write it once, where push and scatter read it twice and write it twice.
 constexpr int NBINS = 9 * TCELLS;
 
-template <typename R>
+template <typename R, int MODE>
 __global__ void __launch_bounds__(256)
-k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
-       const R *psit, size_t cs, R qbm, R dt, R dex, int *bin, int *rank, int *cnt, int *err)
+k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
+       const int *__restrict__ off_new, const R *fxy, const R *bxy, const R *psit, size_t cs,
+       R qbm, R dt, R dex, int *bin, int *rank, int *cnt, int *err, int64_t np)
 {
     __shared__ int hist[NBINS];
     __shared__ R fnode[6][289];   // the six fields at the tile's 17 x 17 nodes
@@ -902,6 +1009,7 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
     const int p0 = off[S * TCELLS], p1 = off[(S + 1) * TCELLS];
     // the particle record of chunk c0 is loaded while chunk c0 - 256 is ranked
     R nx_ = R(0), ny_ = R(0), npx = R(0), npy = R(0), npz = R(0), np6 = R(1), np7 = R(1);
+    R nq = R(0);
     auto fetch = [&](int c) {
         const int i = c + tid;
         if (i < p1) {
@@ -912,6 +1020,7 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
             npz = s.pz[i];
             np6 = s.p6[i];
             np7 = s.p7[i];
+            if (MODE == 2) nq = s.q[i];
         }
     };
     fetch(p0);
@@ -945,9 +1054,10 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
     for (int c0 = p0; c0 < p1; c0 += TCELLS) {
         const int i = c0 + tid;
         const bool valid = i < p1;
-        const R x = nx_, y = ny_, ppx = npx, ppy = npy, ppz = npz, p6 = np6, p7 = np7;
+        const R x = nx_, y = ny_, ppx = npx, ppy = npy, ppz = npz, p6 = np6, p7 = np7, q = nq;
         if (c0 + TCELLS < p1) fetch(c0 + TCELLS);
-        int b = 0;
+        int b = 0, dcell = 0;
+        R ox_ = R(0), oy_ = R(0), opx = R(0), opy = R(0), opz = R(0);
         if (valid) {
             const int cx = min(max((int)x, 0), g.nx - 1), cy = min(max((int)y, 0), g.ny - 1);
             // local node of the cell, clamped into the tile (a particle flagged
@@ -993,9 +1103,14 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
             dx = (rot1 * acx + rot2 * acy + rot3 * acz) * anorm + dx;
             dy = (rot4 * acx + rot5 * acy + rot6 * acz) * anorm + dy;
             dz = (rot7 * acx + rot8 * acy + rot9 * acz) * anorm + dz;
-            s.px[i] = dx;
-            s.py[i] = dy;
-            s.pz[i] = dz;
+            if (MODE == 0) {
+                s.px[i] = dx;
+                s.py[i] = dy;
+                s.pz[i] = dz;
+            }
+            opx = dx;
+            opy = dy;
+            opz = dz;
             R dtc1 = dt / (sqrt(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
             dx = x + dx * dtc1;
             dy = y + dy * dtc1;
@@ -1014,21 +1129,27 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
             if (dx >= anx) dx = dx - anx;
             if (dy < R(0)) dy = dy + any;
             if (dy >= any) dy = dy - any;
-            s.x[i] = dx;
-            s.y[i] = dy;
+            if (MODE == 0) {
+                s.x[i] = dx;
+                s.y[i] = dy;
+            }
+            ox_ = dx;
+            oy_ = dy;
             int ncx = (int)dx, ncy = (int)dy;
             // a non-finite or out-of-range position must not become an index
             if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
-                atomicMax(err, 1 << 30);
+                if (MODE != 2) atomicMax(err, 1 << 30);
                 ncx = min(max(ncx, 0), g.nx - 1);
                 ncy = min(max(ncy, 0), g.ny - 1);
             }
             const int txd = (tx0 + dirx + g.mx1) % g.mx1, tyd = (ty0 + diry + g.my1) % g.my1;
             if ((ncx >> 4) != txd || (ncy >> 4) != tyd) {
-                atomicMax(err, 1 + txd + g.mx1 * tyd);
+                if (MODE != 2) atomicMax(err, 1 + txd + g.mx1 * tyd);
                 dirx = diry = 0;
             }
             b = (3 * (diry + 1) + dirx + 1) * TCELLS + ((ncy & 15) << 4) + (ncx & 15);
+            const int dtx = (tx0 + dirx + g.mx1) % g.mx1, dty = (ty0 + diry + g.my1) % g.my1;
+            dcell = (dty * g.mx1 + dtx) * TCELLS + (b & 255);
         }
         // rank among this chunk's earlier particles of the same bin, in lane order
         unsigned long long todo = __ballot(valid);
@@ -1049,12 +1170,28 @@ k_push(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R 
                 if (nlead) hist[b] += nlead;
             }
         }
-        if (valid) {
+        if (MODE == 0 && valid) {
             bin[i] = S * NBINS + b;
             rank[i] = rk;
         }
+        if (MODE == 2 && valid) {
+            const int64_t d = (int64_t)off_new[dcell] + cnt[(size_t)S * NBINS + b] + rk;
+            if (d >= 0 && d < np) {
+                o.x[d] = ox_;
+                o.y[d] = oy_;
+                o.px[d] = opx;
+                o.py[d] = opy;
+                o.pz[d] = opz;
+                o.p6[d] = p6;
+                o.p7[d] = p7;
+                o.q[d] = q;
+            } else {
+                atomicMax(err, 1 << 29);   // the two passes disagree: never expected
+            }
+        }
         __syncthreads();
     }
+    if (MODE == 2) return;
     __syncthreads();
     for (int k = tid; k < NBINS; k += TCELLS) cnt[(size_t)S * NBINS + k] = hist[k];
 }
@@ -1159,13 +1296,14 @@ static TileGeom geom(qp_ctx *c) { return TileGeom{c->nx, c->ny, c->mx1, c->my1, 
 
 static dim3 pgrid(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-// cell offsets from the per-cell counts
-static int scan_counts(qp_species *sp)
+// cell offsets from the per-cell counts, into off (or the buffer given)
+static int scan_counts(qp_species *sp, DevBuf *dst = nullptr)
 {
     qp_ctx *c = sp->ctx;
     size_t tb = sp->scan_bytes;
     QP_HIP(hipcub::DeviceScan::ExclusiveSum(sp->scan_tmp.p, tb, sp->count.as<int>(),
-                                            sp->off.as<int>(), c->ncells + 1, c->stream));
+                                            (dst ? dst : &sp->off)->as<int>(), c->ncells + 1,
+                                            c->stream));
     return QP_OK;
 }
 
@@ -1187,6 +1325,7 @@ static int check_err(qp_species *sp, const char *what)
     if (h) {
         QP_HIP(hipMemsetAsync(sp->err.p, 0, sizeof(int), c->stream));
         if (h >= (1 << 30)) set_error("%s: particle position outside the grid", what);
+        else if (h >= (1 << 29)) set_error("%s: the push's two passes disagree", what);
         else set_error("%s: particle left its tile by more than one tile (tile %d)", what, h);
         return QP_ERR_CHECK;
     }
@@ -1257,7 +1396,8 @@ struct DepShape {
     int wave;          // 1: k_amjdp_w (independent waves, staging chunk ch per wave);
                        // 2: k_amjdp_pf (next chunk prefetched into registers);
                        // 3: k_amjdp_dma (FP32: chunks copied into LDS by LDS-DMA,
-                       //    double-buffered)
+                       //    double-buffered); 4: k_amjdp_d (independent waves,
+                       //    no staging; rg and ch unused)
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
@@ -1266,7 +1406,8 @@ template <> struct DepTab<float> {
     static constexpr DepShape j[] = {{8, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
-                                     {8, 512, 1, 2},   {8, 512, 1, 3}};
+                                     {8, 512, 1, 2},   {8, 512, 1, 3},  {2, 0, 4, 4},
+                                     {2, 0, 5, 4}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
@@ -1274,7 +1415,8 @@ template <> struct DepTab<double> {
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
                                      {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
-                                     {8, 256, 1, 2},   {8, 256, 1, 0}};
+                                     {8, 256, 1, 2},   {8, 256, 1, 0},  {2, 0, 2, 4},
+                                     {2, 0, 1, 4}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1386,6 +1528,7 @@ static int alloc_particles(qp_species *sp, int64_t np)
     if (!rc) rc = sp->count.alloc((size_t)(c->ncells + 1) * 4);
     if (!rc) rc = sp->bins.alloc((size_t)c->ntiles * NBINS * 4);
     if (!rc) rc = sp->off.alloc((size_t)(c->ncells + 1) * 4);
+    if (!rc) rc = sp->off_new.alloc((size_t)(c->ncells + 1) * 4);
     if (!rc) rc = sp->err.alloc(sizeof(int));
     if (rc) return rc;
     QP_HIP(hipHostMalloc((void **)&sp->h_err, sizeof(int), hipHostMallocDefault));
@@ -1433,6 +1576,9 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
         static_assert(std::is_same<R, float>::value, "LDS-DMA staging is FP32 only");
         lds = (2 * 7 * (size_t)(d.ch + d.ch / 64) + 8 * 289) * sizeof(R);
         kern = k_amjdp_dma<d.rg, d.ch>;
+    } else if constexpr (d.wave == 4) {
+        lds = (size_t)8 * 8 * 51 * sizeof(R);
+        kern = k_amjdp_d<R, d.wpe>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
@@ -1469,6 +1615,8 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out, psi); break;
     case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
     case 10: rc = amjdp_v<R, 10>(sp, ef, bf, psit, dex, out, psi); break;
+    case 11: rc = amjdp_v<R, 11>(sp, ef, bf, psit, dex, out, psi); break;
+    case 12: rc = amjdp_v<R, 12>(sp, ef, bf, psit, dex, out, psi); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;
@@ -1494,12 +1642,18 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
 {
     qp_ctx *c = sp->ctx;
     const TileGeom g = geom(c);
+    const bool scatter = c->tune[QP_TUNE_PUSH] == 1;
+    const double fields = 6.0 * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R);
+    const Soa<R> a = soa<R>(sp->a), b = soa<R>(sp->b);
+    int *off = sp->off.as<int>(), *off_new = sp->off_new.as<int>();
     {
-    ProfScope _ps(c, QP_PROF_PUSH, (12.0 * sp->np + 6.0 * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
-    hipLaunchKernelGGL(k_push<R>, dim3(c->ntiles), dim3(TCELLS), 0, c->stream, g, soa<R>(sp->a),
-                       sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(),
-                       c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex, sp->key.as<int>(),
-                       sp->slot.as<int>(), sp->bins.as<int>(), sp->err.as<int>());
+    // in place: 7 arrays read, 5 written; counting: 7 read
+    ProfScope _ps(c, QP_PROF_PUSH, (scatter ? 12.0 : 7.0) * sp->np * sizeof(R) + fields);
+    auto kern = scatter ? k_push<R, 0> : k_push<R, 1>;
+    hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(TCELLS), 0, c->stream, g, a, b, off, off_new,
+                       ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(), c->rplane,
+                       (R)sp->qbm, (R)sp->dt, (R)dex, sp->key.as<int>(), sp->slot.as<int>(),
+                       sp->bins.as<int>(), sp->err.as<int>(), sp->np);
     }
     QP_LAUNCH_CHECK();
     {
@@ -1508,15 +1662,31 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                        sp->bins.as<int>(), sp->count.as<int>());
     }
     QP_LAUNCH_CHECK();
-    int rc = scan_counts(sp);
-    if (rc) return rc;
-    {
-    ProfScope _ps(c, QP_PROF_SORT, 18.0 * sizeof(R) * (double)sp->np);
-    hipLaunchKernelGGL(k_scatter<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np,
-                       soa<R>(sp->a), soa<R>(sp->b), sp->key.as<int>(), sp->slot.as<int>(),
-                       sp->bins.as<int>(), sp->off.as<int>(), c->mx1, c->my1);
+    if (scatter) {
+        int rc = scan_counts(sp);
+        if (rc) return rc;
+        {
+        ProfScope _ps(c, QP_PROF_SORT, 18.0 * sizeof(R) * (double)sp->np);
+        hipLaunchKernelGGL(k_scatter<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np, a, b,
+                           sp->key.as<int>(), sp->slot.as<int>(), sp->bins.as<int>(), off,
+                           c->mx1, c->my1);
+        }
+        QP_LAUNCH_CHECK();
+    } else {
+        int rc = scan_counts(sp, &sp->off_new);
+        if (rc) return rc;
+        {
+        // 8 arrays read, 8 written at their sorted places
+        ProfScope _ps(c, QP_PROF_SORT, 16.0 * sp->np * sizeof(R) + fields);
+        auto kern = k_push<R, 2>;
+        hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(TCELLS), 0, c->stream, g, a, b,
+                           off, off_new, ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(),
+                           c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex, nullptr, nullptr,
+                           sp->bins.as<int>(), sp->err.as<int>(), sp->np);
+        }
+        QP_LAUNCH_CHECK();
+        std::swap(sp->off, sp->off_new);
     }
-    QP_LAUNCH_CHECK();
     for (int k = 0; k < 8; ++k) std::swap(sp->a[k], sp->b[k]);
     return check ? check_err(sp, "pmove") : QP_OK;
 }
@@ -1566,6 +1736,7 @@ int qp_species_destroy(qp_species *sp)
     if (sp->ctx) (void)hipStreamSynchronize(sp->ctx->stream);
     for (int k = 0; k < 8; ++k) { sp->a[k].release(); sp->b[k].release(); }
     sp->key.release(); sp->slot.release(); sp->count.release(); sp->off.release();
+    sp->off_new.release();
     sp->bins.release();
     sp->scan_tmp.release(); sp->err.release();
     if (sp->h_err) (void)hipHostFree(sp->h_err);
@@ -1780,6 +1951,7 @@ int species_check_poll(qp_species *sp, bool wait)
     *sp->h_err = 0;
     QP_HIP(hipMemsetAsync(sp->err.p, 0, sizeof(int), sp->ctx->stream));
     if (h >= (1 << 30)) set_error("pmove: particle position outside the grid");
+    else if (h >= (1 << 29)) set_error("pmove: the push's two passes disagree");
     else set_error("pmove: particle left its tile by more than one tile (tile %d)", h);
     return QP_ERR_CHECK;
 }

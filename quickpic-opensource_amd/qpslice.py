@@ -34,7 +34,7 @@ FP64, FP32 = 0, 1
  PROF_AMJDP, PROF_PUSH, PROF_SORT, PROF_BORDER, PROF_BEAM_DEPOSIT, PROF_BEAM_PUSH) = range(12)
 PROF_NAMES = ("fft", "transpose", "kspace", "elementwise", "deposit_q", "extpsi", "amjdp",
               "push", "sort", "border", "beam_deposit", "beam_push")
-TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE, TUNE_SEED, TUNE_FFT = 0, 1, 2, 3, 4
+TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE, TUNE_SEED, TUNE_FFT, TUNE_PUSH = 0, 1, 2, 3, 4, 5
 (F_QB, F_QE, F_PSIT, F_PSI, F_DIVVPOT, F_FXY, F_BXYZ, F_CU, F_DCU, F_AMU,
  F_EPW, F_EPWB) = range(12)
 
@@ -142,8 +142,9 @@ class Ctx:
         return ms.value, n.value, b.value
 
     def tune(self, knob, value):
-        """Selects launch shape `value` of a deposit kernel (TUNE_*); -1 is
-        the default.  Shapes change only the order of per-cell sums."""
+        """Selects variant `value` of a tuning knob (TUNE_*); -1 is the
+        default.  Variants change launch shapes, kernel forms or the order of
+        per-cell sums, never what is computed."""
         _call(lib().qp_ctx_tune, self.h, knob, value)
 
     def tune_get(self, knob):
