@@ -61,6 +61,8 @@ def run_gpu(args, precision, beam_part=None):
     # corrector seed: the FP32 leg uses --fp32-seed, the plain FP64 leg the
     # reference's previous-slice field; both iterate to the same tolerance
     ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
+    # corrector iteration: the FP32 leg mixes (--fp32-mix), the FP64 leg is plain
+    ctx.tune(qps.TUNE_MIX, args.fp32_mix if precision == qps.FP32 else 0)
     n = ctx.nx
     t0 = time.time()
     sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
@@ -103,9 +105,10 @@ def run_gpu(args, precision, beam_part=None):
     prof = {qps.PROF_NAMES[k]: ctx.profile_get(k) for k in range(len(qps.PROF_NAMES))}
     ctx.profile(False)
     sweep = {}
+    j_cont = j_first + args.steps + nprof   # next slice of the step
     if args.sweep:
         # the deposit launch shapes, each over the same profiled slices
-        j0 = j_first + args.steps + nprof
+        j0 = j_cont
         for knob, name in ((qps.TUNE_AMJDP, "amjdp"), (qps.TUNE_DEPOSIT_Q, "deposit_q")):
             _, nv = ctx.tune_get(knob)
             kind = qps.PROF_NAMES.index(name)
@@ -142,6 +145,18 @@ def run_gpu(args, precision, beam_part=None):
             sweep[f"seed[{v}] passes"] = float(np.mean(its[2:]))
             j0 += 10
         ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
+        # corrector iteration: plain / Anderson-mixed, wall time and passes
+        for v in (0, 1):
+            ctx.tune(qps.TUNE_MIX, v)
+            ctx.sync()
+            t0 = time.perf_counter()
+            its = [sim.slice(j, iter_max=args.iter_max, tol=args.tol)[0]
+                   for j in range(j0, j0 + 10)]
+            ctx.sync()
+            sweep[f"mix[{v}] ms/slice"] = round((time.perf_counter() - t0) / 10 * 1e3, 4)
+            sweep[f"mix[{v}] passes"] = float(np.mean(its))
+            j0 += 10
+        ctx.tune(qps.TUNE_MIX, args.fp32_mix if precision == qps.FP32 else 0)
         # kernel time per slice of the classes a knob changes (HIP events on
         # the library's stream), so the pass count of the slices drops out
         def class_ms(knob, v, names, nsl=5):
@@ -169,6 +184,7 @@ def run_gpu(args, precision, beam_part=None):
             sweep[f"push[{v}] ms/slice"] = class_ms(qps.TUNE_PUSH, v, ("push", "sort"))[0]
         ctx.tune(qps.TUNE_PUSH, -1)
         log(f"[bench] sweep (ms per launch): {sweep}")
+        j_cont = j0
     # every kernel class: time, launches and algorithmic bytes (operands read
     # or written once) counted by the library at each launch
     classes = {}
@@ -210,7 +226,7 @@ def run_gpu(args, precision, beam_part=None):
     # A particle that crosses more than one tile in a slice stops the step,
     # as PPPCHECK2L stops the reference (part2d_class.f03:582-588); the
     # comparison then covers what was computed.
-    j0 = j_first + args.steps + nprof + (70 if args.sweep else 0)
+    j0 = j_cont
     j_end = min(ctx.nz, int(np.ceil((2.0 + 5 * 0.5) / ctx.dxi)) + 2)
     samples = sorted({j0 + k * (j_end - j0) // 4 for k in range(5)})
     psi = {}
@@ -285,6 +301,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--fp32-seed", type=int, default=1, choices=(0, 1),
                     help="corrector seed of the FP32 leg: 0 previous slice, 1 extrapolated")
+    ap.add_argument("--fp32-mix", type=int, default=0, choices=(0, 1),
+                    help="corrector iteration of the FP32 leg: 0 plain, 1 Anderson-mixed")
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")
@@ -351,8 +369,10 @@ def main():
                         f"nb/n0={args.peak}, B residual {args.tol:g} (max {args.iter_max} "
                         "passes), FP32 corrector seed: "
                         + {0: "previous slice", 1: "linear extrapolation"}[args.fp32_seed]
+                        + {0: "", 1: ", Anderson-mixed passes"}[args.fp32_mix]
                         + "; FP64 leg: previous slice (plain)",
             "fp32_seed": {0: "previous", 1: "extrapolated"}[args.fp32_seed],
+            "fp32_corrector": {0: "plain", 1: "anderson1"}[args.fp32_mix],
             "grid": [2 ** args.ind, 2 ** args.ind, 2 ** args.indz],
             "ppc": args.ppc * args.ppc,
             "plasma_particles": r32["np_plasma"],

@@ -121,9 +121,16 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * particle, then pushes again and stores every particle at its sorted place;
  * 1 pushes in place, keeping each particle's bin and rank, then scatters.
  * Both give the same arrays, bit for bit.
+ * QP_TUNE_MIX = 1 accelerates the corrector of runs to a tolerance (tol > 0)
+ * by Anderson mixing of depth 1: after pass l >= 2 that has not converged,
+ * the next iterate of B_perp is g_l - gamma (g_l - g_{l-1}), g_l the pass's
+ * result and gamma = <f_l, f_l - f_{l-1}> / |f_l - f_{l-1}|^2 over the
+ * changes f = g - (the pass's input).  The stopping test and the result
+ * of the converged pass are the plain iteration's; only the passes to reach
+ * the tolerance change.  Default 0 (the reference's iteration).
  * value -1 restores a knob's default. */
 enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_SEED = 3,
-       QP_TUNE_FFT = 4, QP_TUNE_PUSH = 5, QP_TUNE_COUNT };
+       QP_TUNE_FFT = 4, QP_TUNE_PUSH = 5, QP_TUNE_MIX = 6, QP_TUNE_COUNT };
 int qp_ctx_tune(qp_ctx *ctx, int knob, int value);
 /* current value and the number of values of a knob */
 int qp_ctx_tune_get(const qp_ctx *ctx, int knob, int *value, int *nvalues);

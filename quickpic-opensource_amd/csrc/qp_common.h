@@ -189,25 +189,42 @@ int bfqp_impl(qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu, qp_field2d *bxy,
               double c, double dex, double *resid_out);
 
 constexpr int BFQP_BLOCKS = 1024;   // blocks of the corrector kernels (FP64 partials)
+constexpr int MIX_PARTIALS = 4096;  // offset of the mixing sums in the reduction buffer
 __global__ void k_sum_partials(const double *partial, int nblk, double *out);
-// ResJob, run by the first wave of one block: lane t adds partials t, t+64,
-// ... in order, then the 64 lane sums are combined by a fixed butterfly
-__device__ __forceinline__ void run_res_job(const ResJob &j)
+// Sums of the pairs (p[2i], p[2i+1]), i < n, by the 64 lanes of one wave:
+// lane t adds pairs t, t+64, ... in order, then the 64 lane sums are combined
+// by a fixed butterfly; every lane gets both sums, the same in every caller.
+__device__ __forceinline__ void wave_pair_sum(const double *p, int n, double &a, double &b)
 {
-    if (j.nblk <= 0 || threadIdx.x >= 64) return;
-    const int t = threadIdx.x;
-    double a = 0.0, b = 0.0;
-    for (int i = t; i < j.nblk; i += 64) {
-        a += j.partial[2 * i];
-        b += j.partial[2 * i + 1];
+    const int t = threadIdx.x & 63;
+    a = 0.0;
+    b = 0.0;
+    for (int i = t; i < n; i += 64) {
+        a += p[2 * i];
+        b += p[2 * i + 1];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o);
         b += __shfl_xor(b, o);
     }
+}
+
+// the corrector residual from the summed partials
+__device__ __forceinline__ double residual_of(double num, double den)
+{
+    return den > 0.0 ? sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
+}
+
+// ResJob, run by the first wave of one block
+__device__ __forceinline__ void run_res_job(const ResJob &j)
+{
+    if (j.nblk <= 0 || threadIdx.x >= 64) return;
+    const int t = threadIdx.x;
+    double a, b;
+    wave_pair_sum(j.partial, j.nblk, a, b);
     if (t == 0) {
-        const double res = b > 0.0 ? sqrt(a / b) : (a > 0.0 ? INFINITY : 0.0);
+        const double res = residual_of(a, b);
         j.out[0] = a;
         j.out[1] = b;
         if (j.flag_out)
@@ -222,12 +239,18 @@ __device__ __forceinline__ void run_res_job(const ResJob &j)
 // ---- fused slice-loop kernels (qp_fused.hip) ----
 struct PassFields {
     qp_field2d *cu, *dcu, *amu, *epw, *epwb, *psit, *bxyz;
+    qp_field2d *gprev, *fprev;   // corrector history of the mixed iteration (or null)
 };
 int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
 // qp: one corrector pass; otherwise the predictor (which also clears the
 // converged flag).  With qp the residual of pass l is written to h_res and,
 // when tol > 0 and it is below tol, the converged flag is set.
 int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol);
+// Anderson mixing (depth 1) of the corrector iterate after pass l (f.gprev,
+// f.fprev set; the pass was run by fused_pass with them): unless pass l has
+// converged, epw <- g_l - gamma (g_l - g_{l-1}) and B_perp reassembled; mix =
+// 0 (first pass, last allowed pass) only keeps g_l for the next pass.
+int fused_mix(const PassFields &f, double dex, int l, double tol, bool mix);
 int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
 int fused_seed(qp_field2d *epw, qp_field2d *hist, int extrap);
 int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);

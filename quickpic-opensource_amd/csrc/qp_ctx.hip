@@ -220,7 +220,7 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     if (!rc) rc = c->scr_r.alloc(MAX_BATCH * pl * c->esize);
     if (!rc) rc = c->scr_k.alloc(MAX_BATCH * pl * c->esize);
     if (!rc) rc = c->halo.alloc((size_t)8 * c->ntiles * HALO * c->esize);
-    if (!rc) rc = c->red.alloc(sizeof(double) * 4096);
+    if (!rc) rc = c->red.alloc(sizeof(double) * 8192);
     if (!rc) rc = c->flags.alloc(sizeof(int) * 64);
     if (rc) { qp_ctx_destroy(c); return rc; }
     e = hipHostMalloc((void **)&c->h_res, sizeof(double) * RES_SLOTS, hipHostMallocDefault);
@@ -279,8 +279,8 @@ int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count, do
 
 int qp_ctx_tune(qp_ctx *c, int knob, int value)
 {
-    static const int nvals[QP_TUNE_COUNT] = {4, 13, 2, 3, 5, 2};
-    static const int dflt[QP_TUNE_COUNT] = {0, 0, 0, 0, 0, 0};
+    static const int nvals[QP_TUNE_COUNT] = {4, 11, 2, 3, 5, 2, 2};
+    static const int dflt[QP_TUNE_COUNT] = {0, 0, 0, 0, 0, 0, 0};
     QP_REQUIRE(c && knob >= 0 && knob < QP_TUNE_COUNT, QP_ERR_ARG, "qp_ctx_tune: bad knob");
     if (value < 0) value = dflt[knob];
     QP_REQUIRE(value < nvals[knob], QP_ERR_ARG, "qp_ctx_tune: knob %d has %d values", knob,
@@ -291,7 +291,7 @@ int qp_ctx_tune(qp_ctx *c, int knob, int value)
 
 int qp_ctx_tune_get(const qp_ctx *c, int knob, int *value, int *nvalues)
 {
-    static const int nvals[QP_TUNE_COUNT] = {4, 13, 2, 3, 5, 2};
+    static const int nvals[QP_TUNE_COUNT] = {4, 11, 2, 3, 5, 2, 2};
     QP_REQUIRE(c && knob >= 0 && knob < QP_TUNE_COUNT, QP_ERR_ARG, "qp_ctx_tune_get: bad knob");
     if (value) *value = c->tune[knob];
     if (nvalues) *nvalues = nvals[knob];

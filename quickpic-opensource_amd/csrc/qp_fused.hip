@@ -36,17 +36,20 @@ __global__ void k_psi_fxy(KGrid g, const R *__restrict__ ffg, const R *__restric
 //   bxyz = (-dex epw_y - epwb_y, dex epw_x + epwb_x, dex b_z)
 // Without QP (the predictor) epw is read as it stands.  With QP the FP64
 // squared change and squared new value of epw are summed per block for the
-// corrector's stopping test.
-template <typename R, bool QP>
+// corrector's stopping test.  With MIX the change f = epw_new - epw_old is
+// also kept in fprev, and <f, f - f_prev> and |f - f_prev|^2 (f_prev = 0 on
+// the first pass, hist = 0) are summed per block into mixp for k_mix.
+template <typename R, bool QP, bool MIX>
 __global__ void __launch_bounds__(256)
 k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *__restrict__ cu,
        R *dcu, const R *__restrict__ amu, R *epw, const R *__restrict__ epwb, R *psit, R *bxyz,
-       R aa, R idex, R dex, double *partial, int *flags, const int *skip)
+       R aa, R idex, R dex, double *partial, int *flags, const int *skip, R *fprev, int hist,
+       double *mixp)
 {
-    __shared__ double s_num[256], s_den[256];
+    __shared__ double s_num[256], s_den[256], s_ma[MIX ? 256 : 1], s_mb[MIX ? 256 : 1];
     if (skip && *skip) return;
     if (!QP && blockIdx.x == 0 && threadIdx.x < FLAG_WORDS) flags[threadIdx.x] = 0;
-    double num = 0.0, den = 0.0;
+    double num = 0.0, den = 0.0, ma = 0.0, mb = 0.0;
     const size_t cs = g.cs;
     const int total = (g.nx + 1) * (g.ny + 1);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
@@ -93,6 +96,15 @@ k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *_
             const double q1 = (double)n1 - (double)old1, q2 = (double)n2 - (double)old2;
             num += q1 * q1 + q2 * q2;
             den += (double)n1 * n1 + (double)n2 * n2;
+            if (MIX) {
+                const R f1 = n1 - old1, f2 = n2 - old2;
+                const R p1 = hist ? fprev[o] : R(0), p2 = hist ? fprev[cs + o] : R(0);
+                const double d1 = (double)f1 - (double)p1, d2 = (double)f2 - (double)p2;
+                ma += (double)f1 * d1 + (double)f2 * d2;
+                mb += d1 * d1 + d2 * d2;
+                fprev[o] = f1;
+                fprev[cs + o] = f2;
+            }
             e1 = n1;
             e2 = n2;
         }
@@ -115,17 +127,76 @@ k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *_
     if (!QP) return;
     s_num[threadIdx.x] = num;
     s_den[threadIdx.x] = den;
+    if (MIX) {
+        s_ma[threadIdx.x] = ma;
+        s_mb[threadIdx.x] = mb;
+    }
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             s_num[threadIdx.x] += s_num[threadIdx.x + s];
             s_den[threadIdx.x] += s_den[threadIdx.x + s];
+            if (MIX) {
+                s_ma[threadIdx.x] += s_ma[threadIdx.x + s];
+                s_mb[threadIdx.x] += s_mb[threadIdx.x + s];
+            }
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = s_num[0];
         partial[2 * blockIdx.x + 1] = s_den[0];
+        if (MIX) {
+            mixp[2 * blockIdx.x] = s_ma[0];
+            mixp[2 * blockIdx.x + 1] = s_mb[0];
+        }
+    }
+}
+
+// Anderson mixing of depth 1 after corrector pass l.  With g = epw (the
+// pass's result), g' = gprev (the previous pass's) and gamma = <f, f - f'> /
+// |f - f'|^2 from k_pass's sums, the next iterate is g - gamma (g - g') and
+// B_perp = (-dex e_y - epwb_y, dex e_x + epwb_x) is assembled from it.  Every
+// block sums the partials in the residual's own order, so all blocks, and the
+// host, agree on whether pass l converged: a converged pass is left as it is
+// (the plain iteration's result).  mix = 0 only keeps g for the next pass.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_mix(KGrid g, R *epw, R *gprev, const R *__restrict__ epwb, R *bxyz, R dex,
+      const double *partial, const double *mixp, int nblk, double tol, int mix, const int *skip)
+{
+    __shared__ double s_gam;
+    __shared__ int s_act;
+    if (skip && *skip) return;
+    if (threadIdx.x < 64) {
+        double num, den, a, b;
+        wave_pair_sum(partial, nblk, num, den);
+        wave_pair_sum(mixp, nblk, a, b);
+        if (threadIdx.x == 0) {
+            const bool conv = tol > 0.0 && residual_of(num, den) < tol;
+            s_act = conv ? 0 : (mix ? 2 : 1);   // 0: nothing, 1: keep g, 2: mix
+            s_gam = b > 0.0 ? a / b : 0.0;
+        }
+    }
+    __syncthreads();
+    const int act = s_act;
+    if (act == 0) return;
+    const R gam = (R)s_gam;
+    const size_t cs = g.cs;
+    const int total = (g.nx + 1) * (g.ny + 1);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int kx = t / (g.ny + 1), ky = t - kx * (g.ny + 1);
+        const size_t o = (size_t)kx * g.ldk + ky;
+        const R e1 = epw[o], e2 = epw[cs + o];
+        if (act == 2) {
+            const R x1 = e1 - gam * (e1 - gprev[o]), x2 = e2 - gam * (e2 - gprev[cs + o]);
+            epw[o] = x1;
+            epw[cs + o] = x2;
+            bxyz[o] = x2 * (-dex) - epwb[cs + o];
+            bxyz[cs + o] = x1 * dex + epwb[o];
+        }
+        gprev[o] = e1;
+        gprev[cs + o] = e2;
     }
 }
 
@@ -231,12 +302,15 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol)
     const double planes = qp ? 22.0 : 12.0;
     {
     ProfScope _ps(c, QP_PROF_KSPACE, planes * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R));
-    auto kern = qp ? k_pass<R, true> : k_pass<R, false>;
+    const bool mix = qp && f.fprev;
+    auto kern = mix ? k_pass<R, true, true> : qp ? k_pass<R, true, false> : k_pass<R, false, false>;
     hipLaunchKernelGGL(kern, dim3(BFQP_BLOCKS), dim3(256), 0, c->stream, kgrid(c),
                        c->ffg.as<R>(), c->ffs.as<R>(), f.cu->ks.as<R>(), f.dcu->ks.as<R>(),
                        f.amu->ks.as<R>(), f.epw->ks.as<R>(), f.epwb->ks.as<R>(),
                        f.psit->ks.as<R>(), f.bxyz->ks.as<R>(), (R)(dex * dex), (R)(1.0 / dex),
-                       (R)dex, part, c->flags.as<int>(), qp ? c->skip : nullptr);
+                       (R)dex, part, c->flags.as<int>(), qp ? c->skip : nullptr,
+                       mix ? f.fprev->ks.as<R>() : nullptr, l >= 2 ? 1 : 0,
+                       part + MIX_PARTIALS);
     }
     QP_LAUNCH_CHECK();
     if (qp) {
@@ -255,6 +329,31 @@ int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol)
 {
     return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, l, tol)
                                       : pass_t<float>(f, dex, qp, l, tol);
+}
+
+template <typename R>
+static int mix_t(const PassFields &f, double dex, int l, double tol, bool mix)
+{
+    qp_ctx *c = f.epw->ctx;
+    const double *part = c->red.as<double>();
+    {
+    // epw 2+2, gprev 2+2, epwb 2, bxyz 2
+    ProfScope _ps(c, QP_PROF_KSPACE, 12.0 * (c->nx + 1.0) * (c->ny + 1.0) * sizeof(R));
+    hipLaunchKernelGGL(k_mix<R>, dim3(BFQP_BLOCKS), dim3(256), 0, c->stream, kgrid(c),
+                       f.epw->ks.as<R>(), f.gprev->ks.as<R>(), f.epwb->ks.as<R>(),
+                       f.bxyz->ks.as<R>(), (R)dex, part, part + MIX_PARTIALS, BFQP_BLOCKS, tol,
+                       mix ? 1 : 0, c->skip);
+    }
+    QP_LAUNCH_CHECK();
+    (void)l;
+    return QP_OK;
+}
+
+int fused_mix(const PassFields &f, double dex, int l, double tol, bool mix)
+{
+    QP_REQUIRE(f.gprev && f.fprev, QP_ERR_ARG, "fused_mix: no corrector history");
+    return f.epw->ctx->prec == QP_FP64 ? mix_t<double>(f, dex, l, tol, mix)
+                                       : mix_t<float>(f, dex, l, tol, mix);
 }
 
 template <typename R>

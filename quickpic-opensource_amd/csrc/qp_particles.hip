@@ -875,102 +875,6 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
     }
 }
 
-// The deposit of k_amjdp_w with no staging: each lane reads its particles
-// straight from memory, the record of its next particle loaded while the
-// current one is computed, and writes p6 (with psi also p7) back in place.
-// A wave's 32 cells (one row pair) hold one contiguous run of the sorted
-// arrays, so its loads fall on the few cache lines of that run; no barrier
-// orders the particle loop.  Lanes, cells and the order of the sums are those
-// of k_amjdp_w.
-template <typename R, int WPE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-k_amjdp_d(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
-          const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-          const int *skip, const R *psi)
-{
-    if (skip && *skip) return;
-    constexpr int NR = 3 * 17;   // node rows of a row pair
-    extern __shared__ unsigned char smem_raw[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    R *rnode = reinterpret_cast<R *>(smem_raw);       // [row pair 8][comp 8][3][17]
-    const int tile = blockIdx.x, tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
-    for (int n = tid; n < 8 * 8 * NR; n += TCELLS) rnode[n] = R(0);
-    __syncthreads();
-    const int cl = lane >> 1, sub = lane & 1;
-    const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
-    for (int it = 0; it < 2; ++it) {
-        const int rp = wave + 4 * it;
-        const int lrow = 2 * rp + (cl >> 4), lcol = cl & 15;
-        const int key = kb + lrow * TILE + lcol;
-        const int i0 = off[key], i1 = off[key + 1];
-        const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
-        Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
-        prep_fields(F);
-        const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
-        const R fx = (R)cx, fy = (R)cy;
-        R acc[8][4];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) acc[c][k] = R(0);
-        R x = R(0), y = R(0), px = R(0), py = R(0), p6 = R(1), p7 = R(1), q = R(0);
-        auto load = [&](int i) {
-            x = s.x[i];
-            y = s.y[i];
-            px = s.px[i];
-            py = s.py[i];
-            p6 = s.p6[i];
-            p7 = s.p7[i];
-            q = s.q[i];
-        };
-        int i = i0 + sub;
-        if (i < i1) load(i);
-        for (; i < i1; i += 2) {
-            const R cx_ = x, cy_ = y, cpx = px, cpy = py, cq = q;
-            R c6 = p6, c7 = p7;
-            if (i + 2 < i1) load(i + 2);
-            if (psi) {
-                extpsi_one(FP, fx, fy, cx_, cy_, cpx, cpy, qbm, dex, c6, c7);
-                s.p7[i] = c7;
-            }
-            s.p6[i] = amjdp_one(F, fx, fy, cx_, cy_, cpx, cpy, c6, c7, cq, qtmh, qbm, dti, dex,
-                                idex, acc);
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) acc[c][k] += __shfl_xor(acc[c][k], 1);
-        R *rn = rnode + rp * 8 * NR;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            wave_sync();
-            if (sub == 0) {
-                const int node = ((cl >> 4) + (k >> 1)) * 17 + lcol + (k & 1);
-#pragma unroll
-                for (int c = 0; c < 8; ++c) rn[c * NR + node] += acc[c][k];
-            }
-        }
-    }
-    __syncthreads();
-    for (int n = tid; n < 17 * 17; n += TCELLS) {
-        const int i = n % 17, j = n / 17;
-        const int rp = j >> 1;
-        const bool top = (j & 1) == 0 && j > 0;
-        for (int c = 0; c < 8; ++c) {
-            R v;
-            if (j == 16) v = rnode[(7 * 8 + c) * NR + 2 * 17 + i];
-            else {
-                v = rnode[(rp * 8 + c) * NR + (j & 1) * 17 + i];
-                if (top) v = rnode[((rp - 1) * 8 + c) * NR + 2 * 17 + i] + v;
-            }
-            if (i >= 1 && i < TILE && j >= 1 && j < TILE)
-                dep_store(out, c, (size_t)(ty * TILE + j) * g.ldr + tx * TILE + i, v);
-            else
-                halo[c * halo_cs + (size_t)tile * HALO + halo_index(i, j)] = v;
-        }
-    }
-}
-
 // PPGRBPPUSHF23L_QP (part2d_lib77.f:1066-1408): 3-D Boris push in xi, then the
 // periodic wrap of the reorder (PPPORDER2LB :1936-1977).  One workgroup per
 // source tile walks the tile's particles (a contiguous run of the sorted
@@ -1396,8 +1300,7 @@ struct DepShape {
     int wave;          // 1: k_amjdp_w (independent waves, staging chunk ch per wave);
                        // 2: k_amjdp_pf (next chunk prefetched into registers);
                        // 3: k_amjdp_dma (FP32: chunks copied into LDS by LDS-DMA,
-                       //    double-buffered); 4: k_amjdp_d (independent waves,
-                       //    no staging; rg and ch unused)
+                       //    double-buffered)
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
@@ -1406,8 +1309,7 @@ template <> struct DepTab<float> {
     static constexpr DepShape j[] = {{8, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
-                                     {8, 512, 1, 2},   {8, 512, 1, 3},  {2, 0, 4, 4},
-                                     {2, 0, 5, 4}};
+                                     {8, 512, 1, 2},   {8, 512, 1, 3}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
@@ -1415,8 +1317,7 @@ template <> struct DepTab<double> {
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
                                      {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
-                                     {8, 256, 1, 2},   {8, 256, 1, 0},  {2, 0, 2, 4},
-                                     {2, 0, 1, 4}};
+                                     {8, 256, 1, 2},   {8, 256, 1, 0}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1576,9 +1477,6 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
         static_assert(std::is_same<R, float>::value, "LDS-DMA staging is FP32 only");
         lds = (2 * 7 * (size_t)(d.ch + d.ch / 64) + 8 * 289) * sizeof(R);
         kern = k_amjdp_dma<d.rg, d.ch>;
-    } else if constexpr (d.wave == 4) {
-        lds = (size_t)8 * 8 * 51 * sizeof(R);
-        kern = k_amjdp_d<R, d.wpe>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
@@ -1615,8 +1513,6 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out, psi); break;
     case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
     case 10: rc = amjdp_v<R, 10>(sp, ef, bf, psit, dex, out, psi); break;
-    case 11: rc = amjdp_v<R, 11>(sp, ef, bf, psit, dex, out, psi); break;
-    case 12: rc = amjdp_v<R, 12>(sp, ef, bf, psit, dex, out, psi); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;

@@ -32,6 +32,7 @@ struct qp_sim {
     qp_field2d *f[QP_F_COUNT];
     qp_field3d *bexyz, *bbxyz;
     qp_field2d *epw_hist;   // B_perp of the slice before the last (extrapolated seed)
+    qp_field2d *gprev, *fprev;   // last pass's result and change (mixed corrector)
     int hist_slices;        // slices of this step recorded in the history
     int prepared;           // slice whose first part has run (0: none)
     int last_passes;        // corrector passes of the last finished slice
@@ -62,6 +63,8 @@ int qp_sim_create(qp_ctx *c, qp_species *const *sp, int nsp, qp_beam *const *bm,
     if (!rc) rc = qp_field3d_create(c, 3, &s->bexyz);
     if (!rc) rc = qp_field3d_create(c, 3, &s->bbxyz);
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->epw_hist);
+    if (!rc) rc = qp_field2d_create(c, 2, 1, &s->gprev);
+    if (!rc) rc = qp_field2d_create(c, 2, 1, &s->fprev);
     if (rc) { qp_sim_destroy(s); return rc; }
     *out = s;
     return QP_OK;
@@ -74,6 +77,8 @@ int qp_sim_destroy(qp_sim *s)
     qp_field3d_destroy(s->bexyz);
     qp_field3d_destroy(s->bbxyz);
     qp_field2d_destroy(s->epw_hist);
+    qp_field2d_destroy(s->gprev);
+    qp_field2d_destroy(s->fprev);
     delete s;
     return QP_OK;
 }
@@ -197,12 +202,19 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     const int nsp = (int)s->sp.size();
     const double dd = dex * dex;   // psi *= dex^2 (:670) is applied with the slice writes
     // predictor fields from the carried current (:671-675) and the B assembly
-    const PassFields pf{cu, dcu, amu, epw, epwb, psit, bxyz};
+    const PassFields pf{cu, dcu, amu, epw, epwb, psit, bxyz, nullptr, nullptr};
     qp_field2d *inv[2] = {psit, bxyz};
     const int inv_kinds[2] = {1, 2};
     qp_field2d *fwd[3] = {cu, dcu, amu};
     const int fwd_kinds[3] = {1, 1, 3};
     const bool adaptive = tol > 0.0;
+    // the mixed corrector (QP_TUNE_MIX) applies only to runs to a tolerance
+    const bool mixing = adaptive && c->tune[QP_TUNE_MIX] == 1;
+    PassFields pc = pf;
+    if (mixing) {
+        pc.gprev = s->gprev;
+        pc.fprev = s->fprev;
+    }
     if (adaptive && c->tune[QP_TUNE_SEED] == 1) {
         RC(fused_seed(epw, s->epw_hist, s->hist_slices >= 2));
         ++s->hist_slices;
@@ -227,7 +239,8 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
                                    i == nsp - 1, l == 1 ? psi : nullptr));
         RC(fft_r2k_batch(c, fwd, fwd_kinds, 3));
         cu->state = dcu->state = amu->state = 1;
-        RC(fused_pass(pf, dex, true, l, tol));
+        RC(fused_pass(pc, dex, true, l, tol));
+        if (mixing) RC(fused_mix(pc, dex, l, tol, l >= 2 && l < iter_max));
         RC(fft_k2r_batch(c, inv, inv_kinds, 2));
         psit->state = bxyz->state = 0;
         c->prof_tag = 0;

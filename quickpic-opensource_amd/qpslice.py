@@ -34,7 +34,8 @@ FP64, FP32 = 0, 1
  PROF_AMJDP, PROF_PUSH, PROF_SORT, PROF_BORDER, PROF_BEAM_DEPOSIT, PROF_BEAM_PUSH) = range(12)
 PROF_NAMES = ("fft", "transpose", "kspace", "elementwise", "deposit_q", "extpsi", "amjdp",
               "push", "sort", "border", "beam_deposit", "beam_push")
-TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE, TUNE_SEED, TUNE_FFT, TUNE_PUSH = 0, 1, 2, 3, 4, 5
+TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE, TUNE_SEED, TUNE_FFT, TUNE_PUSH, TUNE_MIX = (
+    0, 1, 2, 3, 4, 5, 6)
 (F_QB, F_QE, F_PSIT, F_PSI, F_DIVVPOT, F_FXY, F_BXYZ, F_CU, F_DCU, F_AMU,
  F_EPW, F_EPWB) = range(12)
 

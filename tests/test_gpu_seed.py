@@ -173,3 +173,41 @@ def test_nonzero_correction_enters_the_seed():
     want = 2.0 * e1 - e2 + t.download(1)
     assert rel(got, want) < 1e-5
     assert rel(got, 2.0 * e1 - e2) > 1e-3   # the correction is not negligible here
+
+
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_mixed_corrector_ends_at_the_plain_fixed_point(precision):
+    """Anderson mixing of the corrector (QP_TUNE_MIX = 1) changes only the
+    iterates between the first pass and the converged one: every slice still
+    ends below the tolerance, with fields that agree with the plain
+    iteration's to the level the tolerance allows."""
+    banner(f"mixed corrector {precision}")
+    prec = getattr(qps, precision)
+    ctx0, sim0 = make(prec)
+    ctx1, sim1 = make(prec)
+    ctx1.tune(qps.TUNE_MIX, 1)
+    p0, p1 = [], []
+    for j in range(1, 25):
+        it0, r0 = sim0.slice(j, iter_max=30, tol=TOL)
+        it1, r1 = sim1.slice(j, iter_max=30, tol=TOL)
+        assert it0 < 30 and it1 < 30 and r0 < TOL and r1 < TOL, (j, it0, it1, r0, r1)
+        p0.append(it0)
+        p1.append(it1)
+    print("passes, plain:", p0, flush=True)
+    print("passes, mixed:", p1, flush=True)
+    for a, b in zip(fields(sim1), fields(sim0)):
+        assert rel(a, b) < 1e-4
+
+
+def test_mixing_leaves_fixed_pass_runs_alone():
+    """Runs with a fixed pass count (tol <= 0, the reference's mode) never
+    mix: fields are bit-identical with the knob on or off."""
+    banner("mixing with fixed passes")
+    ctx0, sim0 = make()
+    ctx1, sim1 = make()
+    ctx1.tune(qps.TUNE_MIX, 1)
+    for j in range(1, 9):
+        sim0.slice(j, iter_max=3, tol=0.0)
+        sim1.slice(j, iter_max=3, tol=0.0)
+    for a, b in zip(fields(sim1), fields(sim0)):
+        assert np.array_equal(a, b)

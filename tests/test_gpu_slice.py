@@ -192,19 +192,6 @@ def test_push_resort_paths_identical(precision):
         assert x.shape == y.shape and np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("variant", [11, 12])
-def test_direct_deposit_run_to_run_identical(variant):
-    """The current deposit that reads particles straight from memory
-    (QP_TUNE_AMJDP 11, 12) keeps a fixed sum order: two FP32 runs give the
-    same fields and particles bit for bit (its parity with the oracle is
-    checked with every other shape)."""
-    banner(f"direct deposit {variant} run-to-run")
-    a = _fp32_run(8, TUNE_AMJDP=variant)
-    b = _fp32_run(8, TUNE_AMJDP=variant)
-    for x, y in zip(a, b):
-        assert np.array_equal(x, y)
-
-
 @pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
 def test_every_deposit_shape_matches_oracle(knob):
     """Every compiled launch shape of the deposits gives the oracle's slice
