@@ -29,23 +29,28 @@ METRIC = ("ξ-slices/s at 1024², 16 ppc; CNN-seeded FP32 vs plain FP64 at same 
           "residual")
 HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(REPO, "profiles", "slice_loop_fp32_1024sq_16ppc.md")
-KERNEL_SYMBOL = {"amjdp": "k_amjdp", "push": "k_push", "extpsi": "k_extpsi",
-                 "deposit_q": "k_deposit_q", "sort": "k_scatter"}
+KERNEL_SYMBOL = {"amjdp": "k_amjdp<{p}", "push": "k_push<{p},1>", "extpsi": "k_extpsi<{p}",
+                 "deposit_q": "k_deposit_q<{p}", "sort": "k_push<{p},2>"}
 
 
 def profiled_traffic(kind, precision_name):
     """HBM bytes per launch of a kernel class as rocprofv3 --pmc measured it
     (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size), from the
-    committed profile summary; None when that summary has no such entry."""
+    committed profile summary: the call-weighted mean over the kernels of
+    the class (e.g. the deposit with and without the psi gather); None when
+    that summary has no such entry."""
     if not os.path.exists(PROFILE):
         return None
-    want = f"qp::{KERNEL_SYMBOL[kind]}<{precision_name}"
+    want = "qp::" + KERNEL_SYMBOL[kind].format(p=precision_name)
+    tot = calls = 0.0
     with open(PROFILE) as fh:
         for line in fh:
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic" and parts[1].startswith(want):
-                return float(parts[2])
-    return None
+            if len(parts) in (3, 4) and parts[0] == "traffic" and parts[1].startswith(want):
+                n = float(parts[3]) if len(parts) == 4 else 1.0
+                tot += float(parts[2]) * n
+                calls += n
+    return tot / calls if calls else None
 
 
 def log(*a):

@@ -281,25 +281,33 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     }
 }
 
-// dst(c)[col][row] = src(c)[row][col] for row < rows, col < cols
-template <typename R>
+// dst(c)[col][row] = src(c)[row][col] for row < rows, col < cols, by TS x TS
+// tiles through LDS; each of the 256 threads moves TS*TS/256 elements, all
+// of its loads issued before the first LDS store
+template <typename R, int TS>
 __global__ void __launch_bounds__(256)
 k_transpose(CompSet<R> cs, int ld_src, int ld_dst, int rows, int cols, const int *skip)
 {
-    __shared__ R tile[32][33];
+    constexpr int RS = 256 / TS, K = TS / RS;   // rows per sweep, sweeps
+    __shared__ R tile[TS][TS + 1];
     if (skip && *skip) return;
     const int comp = blockIdx.z;
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int c0 = blockIdx.x * TS, r0 = blockIdx.y * TS;
+    const int tx = threadIdx.x % TS, ty = threadIdx.x / TS;
     const R *s = cs.src[comp];
     R *d = cs.dst[comp];
-    for (int i = ty; i < 32; i += 8) {
-        int r = r0 + i, c = c0 + tx;
-        if (r < rows && c < cols) tile[i][tx] = s[(size_t)r * ld_src + c];
+    R v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = ty + k * RS, r = r0 + i, c = c0 + tx;
+        v[k] = (r < rows && c < cols) ? s[(size_t)r * ld_src + c] : R(0);
     }
+#pragma unroll
+    for (int k = 0; k < K; ++k) tile[ty + k * RS][tx] = v[k];
     __syncthreads();
-    for (int i = ty; i < 32; i += 8) {
-        int c = c0 + i, r = r0 + tx;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = ty + k * RS, c = c0 + i, r = r0 + tx;
         if (r < rows && c < cols) d[(size_t)c * ld_dst + r] = tile[tx][i];
     }
 }
@@ -373,11 +381,14 @@ template <typename R>
 static int transpose(qp_ctx *c, const CompSet<R> &cs, int ncomp, int ld_src, int ld_dst,
                      int rows, int cols)
 {
-    dim3 grid((cols + 31) / 32, (rows + 31) / 32, ncomp);
+    // 64 x 64 tiles in FP32 (16 elements per thread), 32 x 32 in FP64
+    constexpr int TS = sizeof(R) == 4 ? 64 : 32;
+    dim3 grid((cols + TS - 1) / TS, (rows + TS - 1) / TS, ncomp);
     {
     ProfScope _ps(c, QP_PROF_TRANSPOSE, 2.0 * ncomp * (double)rows * cols * sizeof(R));
-    hipLaunchKernelGGL(k_transpose<R>, grid, dim3(256), 0, c->stream, cs, ld_src, ld_dst, rows,
-                       cols, c->skip);
+    auto kern = k_transpose<R, TS>;
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, cs, ld_src, ld_dst, rows, cols,
+                       c->skip);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;

@@ -55,6 +55,10 @@ struct TileGeom {
     int nx, ny, mx1, my1, ldr;
 };
 
+// v mod m for -m <= v < 2m (a tile index plus a direction in {-1, 0, 1}):
+// the periodic neighbour without an integer division
+__device__ __forceinline__ int wrap1(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
+
 __device__ __forceinline__ int cell_key(int cx, int cy, int mx1)
 {
     return ((cy >> 4) * mx1 + (cx >> 4)) * TCELLS + ((cy & 15) << 4) + (cx & 15);
@@ -132,6 +136,8 @@ template <typename R> static DepOut<R> dep_plain(R *const *planes, int n)
 
 // Sum the tile-border contributions of every border node in a fixed order;
 // with fold, row ny is added into row 0 and cleared (PPNACGUARD2L nvp = 1).
+// One thread per border node and component (one thread taking every
+// component of a node measured 1.5x slower: too few waves per CU).
 template <typename R>
 __global__ void k_deposit_border(TileGeom g, const R *__restrict__ halo, size_t halo_cs,
                                  DepOut<R> out, int fold, const int *skip)
@@ -185,6 +191,12 @@ __device__ __forceinline__ int lpad(int p) { return p + (p >> 5); }
 // reciprocal (1 ulp) in FP32, whose results are compared with FP64 at 1e-5
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+// a / b and sqrt: IEEE in FP64 (the reference's arithmetic); the hardware
+// reciprocal and square root (1 ulp) in FP32
+__device__ __forceinline__ double divq(double a, double b) { return a / b; }
+__device__ __forceinline__ float divq(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sqrtq(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrtq(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <int CH> struct Stage {
     static constexpr int N = CH + CH / 32 + 1;
@@ -444,12 +456,15 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
 
 // PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768): half Boris step with the B_z
 // rotation, deposit of J (3), dJ/dxi (2) and momentum flux (3); p6 updated.
-template <typename R, int RG, int CH, int WPE>
+// PSI = false: a pass without the psi gather (psi is ignored), compiled
+// without its corner registers and branches.
+template <typename R, int RG, int CH, int WPE, bool PSI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
         const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-        const int *skip, const R *psi)
+        const int *skip, const R *psi_arg)
 {
+    const R *psi = PSI ? psi_arg : nullptr;
     if (skip && *skip) return;
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
     extern __shared__ unsigned char smem_raw[];
@@ -890,7 +905,8 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
 // reports) sets *err and is kept in its own tile.
 //
 // MODE 0 stores the pushed particle in place with its bin and rank (k_scatter
-// then moves it).  MODE 1 stores nothing per particle: only the histogram.
+// then moves it).  MODE 1 stores nothing per particle: only the histogram,
+// counted with LDS integer atomics (the same counts in any order; no ranks).
 // MODE 2 runs after MODE 1 and k_bin_starts (cnt then holds each bin's start
 // within its cell) and the scan of the new cell counts into off_new: it
 // repeats the push on the unchanged input, so bins and ranks are those of
@@ -973,7 +989,7 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
                 return Corners<R>{fnode[f][m], fnode[f][m + 1], fnode[f][m + 17], fnode[f][m + 18]};
             };
             const Corners<R> ex = lc(0), ey = lc(1), ez = lc(2), bx = lc(3), by = lc(4), bz = lc(5);
-            R qtmh1 = qtmh / p7;
+            R qtmh1 = divq(qtmh, p7);
             R qtmh2 = qtmh1 * p6;
             R dxp = x - (R)cx, dyp = y - (R)cy;
             R amx = R(1) - dxp, amy = R(1) - dyp;
@@ -992,7 +1008,7 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
             R acx = ppx + dx, acy = ppy + dy, acz = ppz + dz;
             R omxt = qtmh1 * ox, omyt = qtmh1 * oy, omzt = qtmh1 * oz;
             R omt = omxt * omxt + omyt * omyt + omzt * omzt;
-            R anorm = R(2) / (R(1) + omt);
+            R anorm = divq(R(2), R(1) + omt);
             omt = R(0.5) * (R(1) - omt);
             R rot4 = omxt * omyt, rot7 = omxt * omzt, rot8 = omyt * omzt;
             R rot1 = omt + omxt * omxt;
@@ -1015,7 +1031,7 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
             opx = dx;
             opy = dy;
             opz = dz;
-            R dtc1 = dt / (sqrt(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
+            R dtc1 = divq(dt, sqrtq(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
             dx = x + dx * dtc1;
             dy = y + dy * dtc1;
             int dirx = 0, diry = 0;
@@ -1046,14 +1062,27 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
                 ncx = min(max(ncx, 0), g.nx - 1);
                 ncy = min(max(ncy, 0), g.ny - 1);
             }
-            const int txd = (tx0 + dirx + g.mx1) % g.mx1, tyd = (ty0 + diry + g.my1) % g.my1;
+            const int txd = wrap1(tx0 + dirx, g.mx1), tyd = wrap1(ty0 + diry, g.my1);
             if ((ncx >> 4) != txd || (ncy >> 4) != tyd) {
                 if (MODE != 2) atomicMax(err, 1 + txd + g.mx1 * tyd);
                 dirx = diry = 0;
             }
             b = (3 * (diry + 1) + dirx + 1) * TCELLS + ((ncy & 15) << 4) + (ncx & 15);
-            const int dtx = (tx0 + dirx + g.mx1) % g.mx1, dty = (ty0 + diry + g.my1) % g.my1;
+            const int dtx = wrap1(tx0 + dirx, g.mx1), dty = wrap1(ty0 + diry, g.my1);
             dcell = (dty * g.mx1 + dtx) * TCELLS + (b & 255);
+        }
+        if (MODE == 1) {
+            // counting only (integer adds give the same counts in any order):
+            // one LDS atomic per run of equal bins along the lanes, of the
+            // run's length (valid lanes are a prefix of the wave)
+            const int bprev = __shfl_up(b, 1);
+            const bool head = valid && (lane == 0 || bprev != b);
+            const unsigned long long heads = __ballot(head);
+            const int nvalid = __popcll(__ballot(valid));
+            const unsigned long long above = heads & ~((2ull << lane) - 1ull);
+            const int end = above ? __ffsll((long long)above) - 1 : nvalid;
+            if (head) atomicAdd(&hist[b], end - lane);
+            continue;
         }
         // rank among this chunk's earlier particles of the same bin, in lane order
         unsigned long long todo = __ballot(valid);
@@ -1066,7 +1095,8 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
             if (lane == leader) nlead = __popcll(m);
             todo &= ~m;
         }
-        // waves add to the tile histogram in order
+        // waves add to the tile histogram in order (the first barrier also
+        // orders the last wave's update of the chunk before)
         for (int w = 0; w < TCELLS / 64; ++w) {
             __syncthreads();
             if (wave == w) {
@@ -1093,7 +1123,6 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
                 atomicMax(err, 1 << 29);   // the two passes disagree: never expected
             }
         }
-        __syncthreads();
     }
     if (MODE == 2) return;
     __syncthreads();
@@ -1115,7 +1144,7 @@ __global__ void k_bin_starts(TileGeom g, int *cnt, int *count)
     int run = 0;
     for (int nb = 0; nb < 9; ++nb) {
         const int dirx = nb % 3 - 1, diry = nb / 3 - 1;
-        const int sx = (tx - dirx + g.mx1) % g.mx1, sy = (ty - diry + g.my1) % g.my1;
+        const int sx = wrap1(tx - dirx, g.mx1), sy = wrap1(ty - diry, g.my1);
         const size_t k = (size_t)(sy * g.mx1 + sx) * NBINS + nb * TCELLS + cl;
         const int c = cnt[k];
         cnt[k] = run;
@@ -1139,7 +1168,7 @@ __global__ void k_scatter(int64_t np, Soa<R> a, Soa<R> b, const int *bin, const 
     __builtin_amdgcn_sched_barrier(0);   // keep these loads ahead of the lookups
     const int bi = bin[i], rk = rank[i];
     const int S = bi / NBINS, bb = bi - S * NBINS, nb = bb >> 8, cl = bb & 255;
-    const int tx = (S % mx1 + nb % 3 - 1 + mx1) % mx1, ty = (S / mx1 + nb / 3 - 1 + my1) % my1;
+    const int tx = wrap1(S % mx1 + nb % 3 - 1, mx1), ty = wrap1(S / mx1 + nb / 3 - 1, my1);
     const int d = off[(ty * mx1 + tx) * TCELLS + cl] + cnt[bi] + rk;
     b.x[d] = v0;
     b.y[d] = v1;
@@ -1280,6 +1309,7 @@ static int rebin_unsorted(qp_species *sp)
 template <typename R>
 static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
 {
+    QP_REQUIRE(ncomp >= 1 && ncomp <= 8, QP_ERR_ARG, "deposit border: %d components", ncomp);
     TileGeom g = geom(c);
     int n = (c->my1 + 1) * (c->nx + 1) + (c->mx1 + 1) * c->my1 * (TILE - 1);
     {
@@ -1479,7 +1509,8 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
         kern = k_amjdp_dma<d.rg, d.ch>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
-        kern = k_amjdp<R, d.rg, d.ch, d.wpe>;
+        if (psi) kern = k_amjdp<R, d.rg, d.ch, d.wpe, true>;
+        else kern = k_amjdp<R, d.rg, d.ch, d.wpe, false>;
     }
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;

@@ -224,7 +224,8 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     psit->state = bxyz->state = 0;
     // corrector passes (:676-710)
     int *flags = c->flags.as<int>();
-    auto pass = [&](int l) -> int {
+    // a pass is its deposits (head) and its field solve (tail)
+    auto head = [&](int l) -> int {
         c->prof_tag = l;
         if (c->skip) c->skip = flags + (l - 1);
         if (nsp == 0) {
@@ -237,6 +238,12 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         for (int i = 0; i < nsp; ++i)
             RC(species_amjdp_fused(s->sp[i], fxy, bxyz, psit, cu, dcu, amu, dex, i == 0,
                                    i == nsp - 1, l == 1 ? psi : nullptr));
+        c->prof_tag = 0;
+        return QP_OK;
+    };
+    auto tail = [&](int l) -> int {
+        c->prof_tag = l;
+        if (c->skip) c->skip = flags + (l - 1);
         RC(fft_r2k_batch(c, fwd, fwd_kinds, 3));
         cu->state = dcu->state = amu->state = 1;
         RC(fused_pass(pc, dex, true, l, tol));
@@ -245,6 +252,10 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         psit->state = bxyz->state = 0;
         c->prof_tag = 0;
         return QP_OK;
+    };
+    auto pass = [&](int l) -> int {
+        RC(head(l));
+        return tail(l);
     };
     int l = 0;
     double res = -1.0;
@@ -260,10 +271,12 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
             if (res < tol || l == iter_max) break;
         }
     } else {
-        // Pass l+1 is enqueued before the residual of pass l is read back, so
-        // the device never idles on the host; once pass l converges, flag
-        // word l is set and the kernels of the pass queued behind it return
-        // at once, leaving the fields of the converged pass.
+        // The deposits of pass l+1 are enqueued before the residual of pass
+        // l is read back, so the device computes them while the host waits
+        // and decides; once pass l converges, flag word l is set and those
+        // deposits return at once, leaving the fields and particles of the
+        // converged pass.  The field solve of pass l+1 is enqueued only when
+        // pass l has not converged.
         struct SkipGuard {
             qp_ctx *c;
             ~SkipGuard() { c->skip = nullptr; }
@@ -272,15 +285,17 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         RC(pass(1));
         QP_HIP(hipEventRecord(c->pass_ev[1], c->stream));
         for (l = 1;; ++l) {
-            if (l < iter_max) {
-                RC(pass(l + 1));
-                QP_HIP(hipEventRecord(c->pass_ev[(l + 1) & 1], c->stream));
-            }
+            const bool more = l < iter_max;
+            if (more) RC(head(l + 1));
             QP_HIP(hipEventSynchronize(c->pass_ev[l & 1]));
             res = c->h_res[l % RES_SLOTS];
-            if (res < tol || l == iter_max) break;
+            if (res < tol || !more) {
+                if (more) prof_discard(c, l + 1);
+                break;
+            }
+            RC(tail(l + 1));
+            QP_HIP(hipEventRecord(c->pass_ev[(l + 1) & 1], c->stream));
         }
-        if (l < iter_max) prof_discard(c, l + 1);
     }
     if (l > iter_max) l = iter_max;
     // species charge diagnostic of the last pass (cbq, :692-695): it reads

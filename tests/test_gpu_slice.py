@@ -192,6 +192,18 @@ def test_push_resort_paths_identical(precision):
         assert x.shape == y.shape and np.array_equal(x, y)
 
 
+def test_speculative_deposits_change_nothing():
+    """With QP_TUNE_SPECULATE = 1 the deposits of pass l+1 are enqueued
+    before the residual of pass l is read and return at once when pass l
+    has converged: pass counts, fields and particles are those of waiting
+    for each residual, bit for bit."""
+    banner("speculative deposits")
+    a = _fp32_run(12)
+    b = _fp32_run(12, TUNE_SPECULATE=1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
 def test_every_deposit_shape_matches_oracle(knob):
     """Every compiled launch shape of the deposits gives the oracle's slice
