@@ -68,6 +68,9 @@ def run_gpu(args, precision, beam_part=None):
     ctx.tune(qps.TUNE_SEED, args.fp32_seed if precision == qps.FP32 else 0)
     # corrector iteration: the FP32 leg mixes (--fp32-mix), the FP64 leg is plain
     ctx.tune(qps.TUNE_MIX, args.fp32_mix if precision == qps.FP32 else 0)
+    # enqueue the next pass's deposits before each residual read (results
+    # are bit-identical either way)
+    ctx.tune(qps.TUNE_SPECULATE, args.speculate)
     n = ctx.nx
     t0 = time.time()
     sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
@@ -137,7 +140,7 @@ def run_gpu(args, precision, beam_part=None):
             ctx.sync()
             sweep[f"speculate[{v}] ms/slice"] = round((time.perf_counter() - t0) / 5 * 1e3, 4)
             j0 += 5
-        ctx.tune(qps.TUNE_SPECULATE, -1)
+        ctx.tune(qps.TUNE_SPECULATE, args.speculate)
         # corrector seed: previous slice (the reference) / linear extrapolation
         for v in (0, 1):
             ctx.tune(qps.TUNE_SEED, v)
@@ -308,6 +311,8 @@ def main():
                     help="corrector seed of the FP32 leg: 0 previous slice, 1 extrapolated")
     ap.add_argument("--fp32-mix", type=int, default=0, choices=(0, 1),
                     help="corrector iteration of the FP32 leg: 0 plain, 1 Anderson-mixed")
+    ap.add_argument("--speculate", type=int, default=1, choices=(0, 1),
+                    help="enqueue the next corrector pass's deposits before reading a residual")
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")
@@ -378,6 +383,7 @@ def main():
                         + "; FP64 leg: previous slice (plain)",
             "fp32_seed": {0: "previous", 1: "extrapolated"}[args.fp32_seed],
             "fp32_corrector": {0: "plain", 1: "anderson1"}[args.fp32_mix],
+            "speculative_deposits": bool(args.speculate),
             "grid": [2 ** args.ind, 2 ** args.ind, 2 ** args.indz],
             "ppc": args.ppc * args.ppc,
             "plasma_particles": r32["np_plasma"],

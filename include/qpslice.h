@@ -105,7 +105,7 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * differ only in how a cell's particles are split over lanes, so results
  * agree to rounding.  QP_TUNE_SPECULATE = 1 lets qp_slice_run enqueue the
  * deposits of corrector pass l+1 before reading the residual of pass l (they
- * return at once if pass l converged); default 0 waits for each residual.
+ * return at once if pass l converged; the default); 0 waits for each residual.
  * QP_TUNE_SEED = 1 seeds the corrector of a slice run
  * with a residual tolerance by linear extrapolation 2 B(xi-1) - B(xi-2)
  * instead of the reference's B(xi-1) (default 0); QP_TUNE_SEED = 2 keeps the

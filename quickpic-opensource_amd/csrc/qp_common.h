@@ -245,14 +245,16 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
 // qp: one corrector pass; otherwise the predictor (which also clears the
 // converged flag).  With qp the residual of pass l is written to h_res and,
 // when tol > 0 and it is below tol, the converged flag is set.
-int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol);
+// seed (predictor only): 1 records B_perp in ehist, 2 also extrapolates
+// epw <- 2 epw - ehist first (the corrector seed of QP_TUNE_SEED = 1)
+int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int seed = 0,
+               qp_field2d *ehist = nullptr);
 // Anderson mixing (depth 1) of the corrector iterate after pass l (f.gprev,
 // f.fprev set; the pass was run by fused_pass with them): unless pass l has
 // converged, epw <- g_l - gamma (g_l - g_{l-1}) and B_perp reassembled; mix =
 // 0 (first pass, last allowed pass) only keeps g_l for the next pass.
 int fused_mix(const PassFields &f, double dex, int l, double tol, bool mix);
 int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
-int fused_seed(qp_field2d *epw, qp_field2d *hist, int extrap);
 int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
 int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz, qp_field2d *psi,
                       qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd);

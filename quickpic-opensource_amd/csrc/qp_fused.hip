@@ -39,12 +39,16 @@ __global__ void k_psi_fxy(KGrid g, const R *__restrict__ ffg, const R *__restric
 // corrector's stopping test.  With MIX the change f = epw_new - epw_old is
 // also kept in fprev, and <f, f - f_prev> and |f - f_prev|^2 (f_prev = 0 on
 // the first pass, hist = 0) are summed per block into mixp for k_mix.
+// The predictor (no QP) also applies the corrector seed by linear
+// extrapolation in xi to each mode first: seed = 1 keeps epw in ehist, seed
+// = 2 also replaces epw by 2 epw - ehist.  Only the iteration count to a
+// given residual depends on the seed.
 template <typename R, bool QP, bool MIX>
 __global__ void __launch_bounds__(256)
 k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *__restrict__ cu,
        R *dcu, const R *__restrict__ amu, R *epw, const R *__restrict__ epwb, R *psit, R *bxyz,
        R aa, R idex, R dex, double *partial, int *flags, const int *skip, R *fprev, int hist,
-       double *mixp)
+       double *mixp, R *ehist, int seed)
 {
     __shared__ double s_num[256], s_den[256], s_ma[MIX ? 256 : 1], s_mb[MIX ? 256 : 1];
     if (skip && *skip) return;
@@ -62,6 +66,17 @@ k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *_
         const R dkx = dk<R>(g.nx, kx), dky = dk<R>(g.ny, ky);
         const R c1 = cu[o], c2 = cu[cs + o], c3 = cu[2 * cs + o];
         R e1 = epw[o], e2 = epw[cs + o];
+        if (!QP && seed) {
+            const R h1 = ehist[o], h2 = ehist[cs + o];
+            ehist[o] = e1;
+            ehist[cs + o] = e2;
+            if (seed == 2) {
+                e1 = R(2) * e1 - h1;
+                e2 = R(2) * e2 - h2;
+                epw[o] = e1;
+                epw[cs + o] = e2;
+            }
+        }
         if (QP) {
             const R old1 = e1, old2 = e2;
             R n1 = R(0), n2 = R(0);
@@ -200,20 +215,6 @@ k_mix(KGrid g, R *epw, R *gprev, const R *__restrict__ epwb, R *bxyz, R dex,
     }
 }
 
-// Corrector seed by linear extrapolation in xi: epw <- 2 epw - hist with
-// hist <- epw (extrap = 1), or only hist <- epw (extrap = 0).  Only the
-// iteration count to a given residual depends on the seed.
-template <typename R>
-__global__ void k_seed(size_t n, R *epw, R *hist, int extrap)
-{
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
-         e += (size_t)gridDim.x * blockDim.x) {
-        const R e1 = epw[e];
-        if (extrap) epw[e] = R(2) * e1 - hist[e];
-        hist[e] = e1;
-    }
-}
-
 // after the loop (simulation_class.f03:725-726): dcu *= dxi; cu(1:2) -= dcu(1:2)
 template <typename R>
 __global__ void k_predict_cu(size_t n, size_t cs, R *cu, R *dcu, R dxi)
@@ -294,7 +295,8 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy)
 }
 
 template <typename R>
-static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol)
+static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, int seed,
+                  qp_field2d *ehist)
 {
     qp_ctx *c = f.cu->ctx;
     double *part = c->red.as<double>();
@@ -310,7 +312,7 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol)
                        f.psit->ks.as<R>(), f.bxyz->ks.as<R>(), (R)(dex * dex), (R)(1.0 / dex),
                        (R)dex, part, c->flags.as<int>(), qp ? c->skip : nullptr,
                        mix ? f.fprev->ks.as<R>() : nullptr, l >= 2 ? 1 : 0,
-                       part + MIX_PARTIALS);
+                       part + MIX_PARTIALS, seed ? ehist->ks.as<R>() : nullptr, qp ? 0 : seed);
     }
     QP_LAUNCH_CHECK();
     if (qp) {
@@ -325,10 +327,12 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol)
     return QP_OK;
 }
 
-int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol)
+int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int seed,
+               qp_field2d *ehist)
 {
-    return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, l, tol)
-                                      : pass_t<float>(f, dex, qp, l, tol);
+    QP_REQUIRE(!seed || (ehist && !qp), QP_ERR_ARG, "fused_pass: seed without history");
+    return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, l, tol, seed, ehist)
+                                      : pass_t<float>(f, dex, qp, l, tol, seed, ehist);
 }
 
 template <typename R>
@@ -354,26 +358,6 @@ int fused_mix(const PassFields &f, double dex, int l, double tol, bool mix)
     QP_REQUIRE(f.gprev && f.fprev, QP_ERR_ARG, "fused_mix: no corrector history");
     return f.epw->ctx->prec == QP_FP64 ? mix_t<double>(f, dex, l, tol, mix)
                                        : mix_t<float>(f, dex, l, tol, mix);
-}
-
-template <typename R>
-static int seed_t(qp_field2d *epw, qp_field2d *hist, int extrap)
-{
-    qp_ctx *c = epw->ctx;
-    const size_t n = (size_t)epw->dim * c->kplane;
-    {
-    ProfScope _ps(c, QP_PROF_ELEMENTWISE, (extrap ? 4.0 : 2.0) * n * sizeof(R));
-    hipLaunchKernelGGL(k_seed<R>, flat_grid(n), dim3(256), 0, c->stream, n, epw->ks.as<R>(),
-                       hist->ks.as<R>(), extrap);
-    }
-    QP_LAUNCH_CHECK();
-    return QP_OK;
-}
-
-int fused_seed(qp_field2d *epw, qp_field2d *hist, int extrap)
-{
-    return epw->ctx->prec == QP_FP64 ? seed_t<double>(epw, hist, extrap)
-                                     : seed_t<float>(epw, hist, extrap);
 }
 
 template <typename R>

@@ -316,7 +316,6 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R 
     emit_nodes(nacc, out, 0, g.ldr, halo, tile, tx, ty);
 }
 
-// WPGPSIPOST2L_QP (part2d_lib77.f:2207-2284), one thread per particle
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R dex)
@@ -1063,13 +1062,16 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
                 ncy = min(max(ncy, 0), g.ny - 1);
             }
             const int txd = wrap1(tx0 + dirx, g.mx1), tyd = wrap1(ty0 + diry, g.my1);
+            // destination tile: the neighbour in the direction of motion, or,
+            // for a particle the move check flags, its own tile
+            int dt = tyd * g.mx1 + txd;
             if ((ncx >> 4) != txd || (ncy >> 4) != tyd) {
                 if (MODE != 2) atomicMax(err, 1 + txd + g.mx1 * tyd);
                 dirx = diry = 0;
+                dt = S;
             }
             b = (3 * (diry + 1) + dirx + 1) * TCELLS + ((ncy & 15) << 4) + (ncx & 15);
-            const int dtx = wrap1(tx0 + dirx, g.mx1), dty = wrap1(ty0 + diry, g.my1);
-            dcell = (dty * g.mx1 + dtx) * TCELLS + (b & 255);
+            dcell = dt * TCELLS + (b & 255);
         }
         if (MODE == 1) {
             // counting only (integer adds give the same counts in any order):
@@ -1218,6 +1220,16 @@ __global__ void k_gather(int64_t np, Soa<R> a, Soa<R> b, const int *src)
     b.p6[p] = v5;
     b.p7[p] = v6;
     b.q[p] = v7;
+}
+
+// the move-check word into pinned host memory, in stream order (a kernel
+// store instead of a copy-engine transfer between the slice's kernels)
+__global__ void k_post_err(const int *err, int *h_err)
+{
+    if (threadIdx.x == 0) {
+        *h_err = *err;
+        __threadfence_system();
+    }
 }
 
 __global__ void k_zero_int(int *p, int n)
@@ -1861,7 +1873,8 @@ int species_check(qp_species *sp) { return check_err(sp, "pmove"); }
 int species_check_post(qp_species *sp)
 {
     qp_ctx *c = sp->ctx;
-    QP_HIP(hipMemcpyAsync(sp->h_err, sp->err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_post_err, dim3(1), dim3(64), 0, c->stream, sp->err.as<int>(), sp->h_err);
+    QP_LAUNCH_CHECK();
     QP_HIP(hipEventRecord(sp->err_ev, c->stream));
     sp->err_pending = true;
     return QP_OK;

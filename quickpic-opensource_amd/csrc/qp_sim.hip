@@ -215,11 +215,13 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         pc.gprev = s->gprev;
         pc.fprev = s->fprev;
     }
+    // the extrapolated corrector seed is applied by the predictor's kernel
+    int seed = 0;
     if (adaptive && c->tune[QP_TUNE_SEED] == 1) {
-        RC(fused_seed(epw, s->epw_hist, s->hist_slices >= 2));
+        seed = s->hist_slices >= 2 ? 2 : 1;
         ++s->hist_slices;
     }
-    RC(fused_pass(pf, dex, false, 0, 0.0));
+    RC(fused_pass(pf, dex, false, 0, 0.0, seed, s->epw_hist));
     RC(fft_k2r_batch(c, inv, inv_kinds, 2));
     psit->state = bxyz->state = 0;
     // corrector passes (:676-710)
