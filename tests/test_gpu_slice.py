@@ -204,6 +204,35 @@ def test_speculative_deposits_change_nothing():
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+@pytest.mark.parametrize("push", [0, 1])
+def test_move_check_reports_a_tile_jump(precision, push):
+    """A particle that crosses more than one tile in one slice is the
+    condition PPPCHECK2L stops the reference on (part2d_class.f03:582-588).
+    One particle is given px = 1/dex, pz = 300 (about 47 cells in the first
+    slice, with no beam); the slice loop reports QP_ERR_CHECK by the end of
+    the step, on either re-sort path (QP_TUNE_PUSH).  Every index the kernels
+    form from a position is clamped to its tile, so the run itself ends
+    normally."""
+    banner(f"move check {precision} push path {push}")
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=getattr(qps, precision))
+    ctx.tune(qps.TUNE_PUSH, push)
+    sp = qps.Species(ctx, ppc=(2, 2))
+    part = sp.download()
+    k = int(np.argmin(np.abs(part[:, 0] - 2.5) + np.abs(part[:, 1] - 2.5)))
+    part[k, 2] = 1.0 / ctx.dex
+    part[k, 4] = 300.0
+    sp.upload(part)
+    sim = qps.Sim(ctx, [sp], [], iter_=2)
+    sim.begin_step()
+    with pytest.raises(qps.QPError, match="left its tile") as ei:
+        for j in range(1, 3):
+            sim.slice(j)
+        sim.end_step()
+    assert ei.value.code == -5
+    ctx.close()
+
+
 @pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
 def test_every_deposit_shape_matches_oracle(knob):
     """Every compiled launch shape of the deposits gives the oracle's slice
