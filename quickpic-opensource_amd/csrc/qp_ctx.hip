@@ -180,7 +180,7 @@ using namespace qp;
 // values of each tuning knob (qp_ctx_tune) and the value a context starts
 // with: speculative deposits on (bit-identical results, measured 1 % faster
 // at 1024^2, 16 ppc), every other knob at its first value
-static const int TUNE_VALUES[QP_TUNE_COUNT] = {4, 12, 2, 3, 5, 2, 2};
+static const int TUNE_VALUES[QP_TUNE_COUNT] = {4, 11, 2, 3, 5, 2, 2};
 static const int TUNE_DEFAULT[QP_TUNE_COUNT] = {0, 0, 1, 0, 0, 0, 0};
 
 extern "C" {

@@ -456,11 +456,10 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
 // PPGRDCJPPOST2L_QP (part2d_lib77.f:311-768): half Boris step with the B_z
 // rotation, deposit of J (3), dJ/dxi (2) and momentum flux (3); p6 updated.
 // PSI = false: a pass without the psi gather (psi is ignored), compiled
-// without its corner registers and branches.  (Spreading a chunk's cells
-// over all four waves, at half their lanes, measured 2 % slower.)  DW: each
-// lane stores its particles' new p6 (and p7) straight to memory, so a chunk
-// needs no barrier and copy-back after its compute.
-template <typename R, int RG, int CH, int WPE, bool PSI, bool DW = false>
+// without its corner registers and branches.  (Measured slower: spreading a
+// chunk's cells over all four waves at half their lanes, 2 %; each lane
+// storing its particles' new p6 itself instead of the copy-back, 1.6 %.)
+template <typename R, int RG, int CH, int WPE, bool PSI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
         const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
@@ -509,20 +508,15 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
                 R p6 = sp6[l], p7 = sp7[l];
                 if (psi) {
                     extpsi_one(FP, fx, fy, sx[l], sy[l], spx[l], spy[l], qbm, dex, p6, p7);
-                    if (DW) s.p7[i] = p7;
-                    else sp7[l] = p7;
+                    sp7[l] = p7;
                 }
-                const R p6n = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l],
-                                        qtmh, qbm, dti, dex, idex, acc);
-                if (DW) s.p6[i] = p6n;
-                else sp6[l] = p6n;
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l], qtmh,
+                                   qbm, dti, dex, idex, acc);
             }
-            if (!DW) {
-                __syncthreads();
-                for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
-                if (psi)
-                    for (int p = tid; p < n; p += TCELLS) s.p7[c0 + p] = sp7[lpad(p)];
-            }
+            __syncthreads();
+            for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
+            if (psi)
+                for (int p = tid; p < n; p += TCELLS) s.p7[c0 + p] = sp7[lpad(p)];
         }
 #pragma unroll
         for (int o = T / 2; o > 0; o >>= 1)
@@ -1350,8 +1344,7 @@ struct DepShape {
     int wave;          // current deposit: 1 k_amjdp_w (independent waves, staging
                        // chunk ch per wave); 2 k_amjdp_pf (next chunk prefetched
                        // into registers); 3 k_amjdp_dma (FP32: chunks copied into
-                       // LDS by LDS-DMA, double-buffered); 4 k_amjdp storing
-                       // p6 (p7) from the computing lane (DW)
+                       // LDS by LDS-DMA, double-buffered)
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
@@ -1360,7 +1353,7 @@ template <> struct DepTab<float> {
     static constexpr DepShape j[] = {{8, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
-                                     {8, 512, 1, 2},   {8, 512, 1, 3},  {8, 1024, 4, 4}};
+                                     {8, 512, 1, 2},   {8, 512, 1, 3}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
@@ -1368,7 +1361,7 @@ template <> struct DepTab<double> {
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
                                      {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
-                                     {8, 256, 1, 2},   {8, 256, 1, 0},  {8, 1024, 1, 4}};
+                                     {8, 256, 1, 2},   {8, 256, 1, 0}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1528,10 +1521,6 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
         static_assert(std::is_same<R, float>::value, "LDS-DMA staging is FP32 only");
         lds = (2 * 7 * (size_t)(d.ch + d.ch / 64) + 8 * 289) * sizeof(R);
         kern = k_amjdp_dma<d.rg, d.ch>;
-    } else if constexpr (d.wave == 4) {
-        lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
-        if (psi) kern = k_amjdp<R, d.rg, d.ch, d.wpe, true, true>;
-        else kern = k_amjdp<R, d.rg, d.ch, d.wpe, false, true>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         if (psi) kern = k_amjdp<R, d.rg, d.ch, d.wpe, true>;
@@ -1569,7 +1558,6 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out, psi); break;
     case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
     case 10: rc = amjdp_v<R, 10>(sp, ef, bf, psit, dex, out, psi); break;
-    case 11: rc = amjdp_v<R, 11>(sp, ef, bf, psit, dex, out, psi); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;
