@@ -30,6 +30,10 @@ struct qp_species {
     qp::DevBuf bins;           // per tile: counts, then starts, of the 9 x 256 destination bins
     qp::DevBuf count, off;     // per cell (ncells + 1)
     qp::DevBuf off_new;        // the push's new cell offsets, swapped with off
+    // every particle has charge q_value (a uniform plasma, or an upload that
+    // says so): q is then filled in both buffers and not moved, staged or read
+    bool q_uniform;
+    double q_value;
     qp::DevBuf scan_tmp, err;
     size_t scan_bytes;
     qp_field2d *q, *qn, *cu, *dcu, *amu;
@@ -255,9 +259,10 @@ __device__ __forceinline__ void emit_nodes(const R *nacc, const DepOut<R> &out, 
 // atomic touches the deposit.
 
 // PPGPPOST2L (part2d_lib77.f:194-309): q(n,m) += qm (1-dx)(1-dy) etc.
-template <typename R, int RG, int CH, int WPE>
+// UQ: every particle's charge is quni (not staged or read).
+template <typename R, int RG, int CH, int WPE, bool UQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R *halo)
+k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R *halo, R quni)
 {
     constexpr int T = TILE / RG, NP = Stage<CH>::N;
     extern __shared__ unsigned char smem_raw[];
@@ -276,7 +281,11 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R 
         for (int c0 = G0; c0 < G1; c0 += CH) {
             const int n = min(CH, G1 - c0);
             __syncthreads();
-            {
+            if (UQ) {
+                const R *src[2] = {s.x, s.y};
+                R *dst[2] = {sx, sy};
+                stage_in<R, 2, CH / TCELLS>(src, dst, c0, n, tid, TCELLS);
+            } else {
                 const R *src[3] = {s.x, s.y, s.q};
                 R *dst[3] = {sx, sy, sq};
                 stage_in<R, 3, CH / TCELLS>(src, dst, c0, n, tid, TCELLS);
@@ -287,7 +296,7 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R 
             const int b = min(i1, c0 + n);
             for (int i = a; i < b; i += T) {
                 const int l = lpad(i - c0);
-                R qm = sq[l];
+                R qm = UQ ? quni : sq[l];
                 R dxp = qm * (sx[l] - fx);
                 R dyp = sy[l] - fy;
                 R amx = qm - dxp;
@@ -459,11 +468,12 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
 // without its corner registers and branches.  (Measured slower: spreading a
 // chunk's cells over all four waves at half their lanes, 2 %; each lane
 // storing its particles' new p6 itself instead of the copy-back, 1.6 %.)
-template <typename R, int RG, int CH, int WPE, bool PSI>
+// UQ: every particle's charge is quni (not staged).
+template <typename R, int RG, int CH, int WPE, bool PSI, bool UQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
         const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-        const int *skip, const R *psi_arg)
+        const int *skip, const R *psi_arg, R quni)
 {
     const R *psi = PSI ? psi_arg : nullptr;
     if (skip && *skip) return;
@@ -494,7 +504,11 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         for (int c0 = G0; c0 < G1; c0 += CH) {
             const int n = min(CH, G1 - c0);
             __syncthreads();
-            {
+            if (UQ) {
+                const R *src[6] = {s.x, s.y, s.px, s.py, s.p6, s.p7};
+                R *dst[6] = {sx, sy, spx, spy, sp6, sp7};
+                stage_in<R, 6, CH / TCELLS>(src, dst, c0, n, tid, TCELLS);
+            } else {
                 const R *src[7] = {s.x, s.y, s.px, s.py, s.p6, s.p7, s.q};
                 R *dst[7] = {sx, sy, spx, spy, sp6, sp7, sq};
                 stage_in<R, 7, CH / TCELLS>(src, dst, c0, n, tid, TCELLS);
@@ -510,8 +524,8 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
                     extpsi_one(FP, fx, fy, sx[l], sy[l], spx[l], spy[l], qbm, dex, p6, p7);
                     sp7[l] = p7;
                 }
-                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7, sq[l], qtmh,
-                                   qbm, dti, dex, idex, acc);
+                sp6[l] = amjdp_one(F, fx, fy, sx[l], sy[l], spx[l], spy[l], p6, p7,
+                                   UQ ? quni : sq[l], qtmh, qbm, dti, dex, idex, acc);
             }
             __syncthreads();
             for (int p = tid; p < n; p += TCELLS) s.p6[c0 + p] = sp6[lpad(p)];
@@ -551,7 +565,7 @@ template <typename R, int RG, int CH>
 __global__ void __launch_bounds__(256)
 k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
            const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-           const int *skip, const R *psi)
+           const int *skip, const R *psi, R)
 {
     if (skip && *skip) return;
     constexpr int T = TILE / RG, NP = Stage<CH>::N, PF = CH / TCELLS;
@@ -675,7 +689,8 @@ template <int RG, int CH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_amjdp_dma(TileGeom g, Soa<float> s, const int *__restrict__ off, const float *fxy,
             const float *bxy, const float *psit, size_t cs, DepOut<float> out, float *halo,
-            size_t halo_cs, float qbm, float dt, float dex, const int *skip, const float *psi)
+            size_t halo_cs, float qbm, float dt, float dex, const int *skip, const float *psi,
+            float)
 {
     using R = float;
     if (skip && *skip) return;
@@ -798,7 +813,7 @@ template <typename R, int CHW>
 __global__ void __launch_bounds__(256)
 k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
           const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
-          const int *skip, const R *psi)
+          const int *skip, const R *psi, R)
 {
     if (skip && *skip) return;
     constexpr int NP = Stage<CHW>::N, NR = 3 * 17;   // node rows of a row pair
@@ -920,7 +935,7 @@ template <typename R, int MODE>
 __global__ void __launch_bounds__(256)
 k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
        const int *__restrict__ off_new, const R *fxy, const R *bxy, const R *psit, size_t cs,
-       R qbm, R dt, R dex, int *bin, int *rank, int *cnt, int *err, int64_t np)
+       R qbm, R dt, R dex, int *bin, int *rank, int *cnt, int *err, int64_t np, int uq)
 {
     __shared__ int hist[NBINS];
     __shared__ R fnode[6][289];   // the six fields at the tile's 17 x 17 nodes
@@ -941,7 +956,7 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
             npz = s.pz[i];
             np6 = s.p6[i];
             np7 = s.p7[i];
-            if (MODE == 2) nq = s.q[i];
+            if (MODE == 2 && !uq) nq = s.q[i];
         }
     };
     fetch(p0);
@@ -1122,7 +1137,7 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
                 o.pz[d] = opz;
                 o.p6[d] = p6;
                 o.p7[d] = p7;
-                o.q[d] = q;
+                if (!uq) o.q[d] = q;   // a uniform q is already in place
             } else {
                 atomicMax(err, 1 << 29);   // the two passes disagree: never expected
             }
@@ -1382,13 +1397,16 @@ static int deposit_q_v(qp_species *sp, const DepOut<R> &out)
     qp_ctx *c = sp->ctx;
     constexpr DepShape d = DepTab<R>::q[V];
     const size_t lds = (3 * (size_t)Stage<d.ch>::N + 289) * sizeof(R);
-    auto kern = k_deposit_q<R, d.rg, d.ch, d.wpe>;
+    auto kern = sp->q_uniform ? k_deposit_q<R, d.rg, d.ch, d.wpe, true>
+                              : k_deposit_q<R, d.rg, d.ch, d.wpe, false>;
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
     {
-    ProfScope _ps(c, QP_PROF_DEPOSIT_Q, (3.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
+    // x, y (and q unless uniform) per particle, one plane
+    const double words = sp->q_uniform ? 2.0 : 3.0;
+    ProfScope _ps(c, QP_PROF_DEPOSIT_Q, (words * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
-                       soa<R>(sp->a), sp->off.as<int>(), out, c->halo.as<R>());
+                       soa<R>(sp->a), sp->off.as<int>(), out, c->halo.as<R>(), (R)sp->q_value);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -1416,6 +1434,18 @@ static int deposit_q_t(qp_species *sp, R *plane)
 
 // dist2d_000 (fdist2d_class.f03:212-316), one rank: x in [1,nx-1),
 // y in [1,ny); record x, y, 0, 0, 0, 1, 1, qm in the reference's order.
+// Marks the species' charge uniform (or not); a uniform q is written into
+// both particle buffers, so no permutation has to move it
+static int set_q_uniform(qp_species *sp, bool uniform, double v)
+{
+    sp->q_uniform = uniform;
+    sp->q_value = v;
+    if (!uniform || sp->np == 0) return QP_OK;
+    int rc = launch_fill(sp->ctx, sp->a[7].p, (size_t)sp->np, v);
+    if (!rc) rc = launch_fill(sp->ctx, sp->b[7].p, (size_t)sp->np, v);
+    return rc;
+}
+
 template <typename R>
 static int distribute(qp_species *sp)
 {
@@ -1443,7 +1473,9 @@ static int distribute(qp_species *sp)
         int rc = launch_fill(c, sp->a[k].p, (size_t)np, fillv[k - 2]);
         if (rc) return rc;
     }
-    return rebin_unsorted<R>(sp);
+    int rc = rebin_unsorted<R>(sp);
+    if (rc) return rc;
+    return set_q_uniform(sp, true, qm);
 }
 
 // renew: particles and qn = -(folded deposit of the fresh plasma)
@@ -1510,7 +1542,7 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     constexpr DepShape d = DepTab<R>::j[V];
     size_t lds;
     void (*kern)(TileGeom, Soa<R>, const int *, const R *, const R *, const R *, size_t,
-                 DepOut<R>, R *, size_t, R, R, R, const int *, const R *);
+                 DepOut<R>, R *, size_t, R, R, R, const int *, const R *, R);
     if constexpr (d.wave == 1) {
         lds = (4 * 7 * (size_t)Stage<d.ch>::N + 8 * 8 * 51) * sizeof(R);
         kern = k_amjdp_w<R, d.ch>;
@@ -1523,19 +1555,25 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
         kern = k_amjdp_dma<d.rg, d.ch>;
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
-        if (psi) kern = k_amjdp<R, d.rg, d.ch, d.wpe, true>;
-        else kern = k_amjdp<R, d.rg, d.ch, d.wpe, false>;
+        if (sp->q_uniform)
+            kern = psi ? k_amjdp<R, d.rg, d.ch, d.wpe, true, true>
+                       : k_amjdp<R, d.rg, d.ch, d.wpe, false, true>;
+        else
+            kern = psi ? k_amjdp<R, d.rg, d.ch, d.wpe, true, false>
+                       : k_amjdp<R, d.rg, d.ch, d.wpe, false, false>;
     }
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
     {
-    // with psi: one more word per particle (p7 written) and one more plane
-    const double words = psi ? 9.0 : 8.0, planes = psi ? 15.0 : 14.0;
+    // with psi: one more word per particle (p7 written) and one more plane;
+    // a uniform q is not read (by the kernels with a uniform-charge form)
+    const double words = (psi ? 9.0 : 8.0) - (sp->q_uniform && d.wave == 0 ? 1.0 : 0.0);
+    const double planes = psi ? 15.0 : 14.0;
     ProfScope _ps(c, QP_PROF_AMJDP, (words * sp->np + planes * (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
                        soa<R>(sp->a), sp->off.as<int>(), ef->rs.as<R>(), bf->rs.as<R>(),
                        psit->rs.as<R>(), pl, out, c->halo.as<R>(), hcs, (R)sp->qbm, (R)sp->dt,
-                       (R)dex, c->skip, psi ? psi->rs.as<R>() : nullptr);
+                       (R)dex, c->skip, psi ? psi->rs.as<R>() : nullptr, (R)sp->q_value);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -1594,7 +1632,7 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(TCELLS), 0, c->stream, g, a, b, off, off_new,
                        ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(), c->rplane,
                        (R)sp->qbm, (R)sp->dt, (R)dex, sp->key.as<int>(), sp->slot.as<int>(),
-                       sp->bins.as<int>(), sp->err.as<int>(), sp->np);
+                       sp->bins.as<int>(), sp->err.as<int>(), sp->np, 0);
     }
     QP_LAUNCH_CHECK();
     {
@@ -1618,12 +1656,12 @@ static int push_t(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
         if (rc) return rc;
         {
         // 8 arrays read, 8 written at their sorted places
-        ProfScope _ps(c, QP_PROF_SORT, 16.0 * sp->np * sizeof(R) + fields);
+        ProfScope _ps(c, QP_PROF_SORT, (sp->q_uniform ? 14.0 : 16.0) * sp->np * sizeof(R) + fields);
         auto kern = k_push<R, 2>;
         hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(TCELLS), 0, c->stream, g, a, b,
                            off, off_new, ef->rs.as<R>(), bf->rs.as<R>(), psit->rs.as<R>(),
                            c->rplane, (R)sp->qbm, (R)sp->dt, (R)dex, nullptr, nullptr,
-                           sp->bins.as<int>(), sp->err.as<int>(), sp->np);
+                           sp->bins.as<int>(), sp->err.as<int>(), sp->np, sp->q_uniform ? 1 : 0);
         }
         QP_LAUNCH_CHECK();
         std::swap(sp->off, sp->off_new);
@@ -1809,7 +1847,13 @@ static int upload_t(qp_species *sp, const double *part, int64_t np)
         QP_HIP(hipStreamSynchronize(c->stream));
     }
     sp->np = np;
-    return rebin_unsorted<R>(sp);
+    // a uniform charge (compared as stored) lets the kernels skip q
+    bool uniform = true;
+    const R q0 = np ? (R)part[7] : R(0);
+    for (int64_t i = 1; uniform && i < np; ++i) uniform = (R)part[7 + 8 * i] == q0;
+    int rc = rebin_unsorted<R>(sp);
+    if (rc) return rc;
+    return set_q_uniform(sp, uniform, (double)q0);
 }
 
 extern "C" {

@@ -272,6 +272,63 @@ def test_species_extpsi_amjdp_push(ctx, sp_pair):
     assert rel(canon(got), canon(ref)) < tol(ctx)
 
 
+def test_uniform_charge_paths(ctx, sp_pair):
+    """Particles that all carry one charge (a uniform plasma, or an upload of
+    one) take the kernels' uniform-charge forms, which neither stage nor move
+    q: the charge deposit, the current deposit and the push still match the
+    oracle, and q downloads intact after the re-sort."""
+    banner("uniform-charge kernels")
+    sp, part = sp_pair
+    rng = np.random.default_rng(13)
+    p = _jitter(rng, part)
+    p[:, 7] = -0.25
+    sp.upload(p)
+    q = qps.Field2d(ctx, 1)
+    q.as_(0.0)
+    sp.qdp(q)
+    mine = np.zeros((1, NY + 1, NX + 1))
+    qpo.qdeposit(NX, NY, p, mine)
+    qpo.guard_fold(mine)
+    qn = np.zeros((1, NY + 1, NX + 1))
+    qpo.qdeposit(NX, NY, part, qn)
+    qpo.guard_fold(qn)
+    assert rel(q.download(0), -qn + mine) < tol(ctx) * 10
+    ref = canon(sp.download())
+    fxy, bxy, psit = smooth(rng, 2), smooth(rng, 3), smooth(rng, 1)
+    F = {}
+    for name, arr in (("fxy", fxy), ("bxy", bxy), ("psit", psit)):
+        F[name] = qps.Field2d(ctx, arr.shape[0])
+        F[name].upload(arr, space=0)
+    cu, dcu, amu = (qps.Field2d(ctx, d) for d in (3, 2, 3))
+    for f in (cu, dcu, amu):
+        f.as_(0.0)
+    sp.amjdp(F["fxy"], F["bxy"], F["psit"], cu, amu, dcu, DEX)
+    rcu, rdcu, ramu = (np.zeros((d, NY + 1, NX + 1)) for d in (3, 2, 3))
+    qpo.amjdp(ref, fxy, bxy, psit, rcu, rdcu, ramu, -1.0, DXI, DEX)
+    for a in (rcu, rdcu, ramu):
+        qpo.guard_fold(a)
+    assert rel(cu.download(0), rcu) < tol(ctx) * 10
+    assert rel(dcu.download(0), rdcu) < tol(ctx) * 10
+    assert rel(amu.download(0), ramu) < tol(ctx) * 10
+    # the push does not read q: the same records in the same order, once with
+    # the uniform charge and once tagged (the general path, held to the
+    # oracle), end bit-identical but for q
+    pre = sp.download()
+    tagged = pre.copy()
+    tagged[:, 7] = -0.25 - np.arange(len(pre)) * 2.0 ** -18
+    sp2 = qps.Species(ctx, ppc=(2, 2), dt=DXI)
+    sp2.upload(tagged)
+    sp.push(F["fxy"], F["bxy"], F["psit"], DEX)
+    sp2.push(F["fxy"], F["bxy"], F["psit"], DEX)
+    got, got2 = sp.download(), sp2.download()
+    assert np.all(got[:, 7] == -0.25)
+    assert np.array_equal(got[:, :7], got2[:, :7])
+    ref = canon(tagged)
+    assert qpo.push(ref, fxy, bxy, psit, -1.0, DXI, DEX) == 0
+    assert rel(canon(got2), canon(ref)) < tol(ctx)
+    sp2.close()
+
+
 def test_beam_loader_and_deposit(ctx):
     banner("beam loader/deposit")
     grid = qpo.Grid(IND, IND, 6)
