@@ -276,8 +276,9 @@ def calib_copy(args):
 
 def cpu_baseline(args):
     """The oracle (FP64, one thread) on a bounded sample of the same workload:
-    two xi slices at the full transverse size and ppc, 2 corrector passes,
-    no beam (the per-slice CPU cost does not depend on the beam)."""
+    --cpu-slices xi slices (about 10 s) at the full transverse size and ppc,
+    2 corrector passes, no beam (the per-slice CPU cost does not depend on
+    the beam)."""
     from oracle import qpo
     t0 = time.time()
     grid = qpo.Grid(args.ind, args.ind, 4)
@@ -316,7 +317,7 @@ def main():
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")
-    ap.add_argument("--cpu-slices", type=int, default=2)
+    ap.add_argument("--cpu-slices", type=int, default=4)
     ap.add_argument("--prof-steps", type=int, default=10,
                     help="slices after the timed ones run with per-launch events")
     ap.add_argument("--sweep", action="store_true",
