@@ -151,7 +151,6 @@ int qp_slice_prepare(qp_sim *s, int j)
     qp_ctx *c = s->ctx;
     QP_REQUIRE(j >= 1 && j <= c->nz, QP_ERR_ARG, "qp_slice_prepare: slice %d outside 1..%d", j,
                c->nz);
-    const double dex = c->dex;
     qp_field2d **F = s->f;
     qp_field2d *qb = F[QP_F_QB], *qe = F[QP_F_QE], *psi = F[QP_F_PSI], *fxy = F[QP_F_FXY];
     qp_field2d *epwb = F[QP_F_EPWB];
