@@ -114,9 +114,10 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * (tol <= 0) use the reference's seed whatever the knob.
  * QP_TUNE_FFT = 1 folds the transpose between the x and y passes of every
  * transform into the y pass (column-pair reads / writes) instead of a
- * separate LDS-tiled transpose (default 1 in FP64, 0 in FP32); 0, 2, 3, 4 keep the
- * transpose and pad the transforms' LDS by one point per 8, 32 or 4 points
- * (default per 16 in FP32, per 8 in FP64); results are identical.
+ * separate LDS-tiled transpose (the FP64 default).  0 (the FP32 default),
+ * 2, 3 and 4 keep the transpose and pad the transforms' LDS by one point per
+ * 16 points in FP32 or 8 in FP64 (0), or per 8, 32 or 4 points (2, 3, 4).
+ * Results are identical.
  * QP_TUNE_PUSH selects how the plasma push re-sorts the particles: 0 (default)
  * counts the destination bins in a first pass that stores nothing per
  * particle, then pushes again and stores every particle at its sorted place;
