@@ -59,8 +59,10 @@ def test_bfqp(g):
 
 
 def test_plasma_loader_matches_the_stored_input(g):
-    # dist2d_000 (fdist2d_class.f03:212-316): the stored input was made by
-    # the oracle's loader; its positions are the quiet-start lattice of Q1
+    # dist2d_000 (fdist2d_class.f03:212-316).  The stored input was made by
+    # the oracle's loader, so this only guards the fixture against drift; the
+    # loader itself is pinned against the independently written quiet-start
+    # lattice in test_oracle_physics.py
     nx, ny = int(g["nx"]), int(g["ny"])
     part = qpo.plasma_init(nx, ny, 2, 2)
     qpo.order_by_tile(nx, ny, part)

@@ -245,3 +245,60 @@ def test_beam_deposit_and_push(ref):
     theirs = ref.beam_push(part.copy(), e3r, b3r, -1.0, dt, grid.dx, grid.dz, nx, ny, nz)
     assert 0 < mine.shape[0] < part.shape[0]
     assert np.array_equal(mine, theirs)  # same survivors, same order
+
+
+def _composed(indx, indy, species, beams, nslices):
+    from oracle import refslice
+    grid = qpo.Grid(indx, indy, 5)
+    osim = qpo.Simulation(grid, species, beams, dt=10.0, iter_=2)
+    rsim = refslice.RefSimulation(fresh_ref(), grid, species, beams, dt=10.0, iter_=2)
+    osim.begin_step()
+    rsim.begin_step()
+    assert np.array_equal(osim.q3[0], np.transpose(rsim.q3[:grid.nx + 1], (2, 1, 0))) or \
+        rel(osim.q3[0], np.transpose(rsim.q3[:grid.nx + 1], (2, 1, 0))) < 1e-14
+    for j in range(1, nslices + 1):
+        osim.slice(j)
+        rsim.slice(j)
+        for c in range(3):
+            # Bz is a small second-order field built from cancelling terms:
+            # its bar is relative to its own peak, like the others
+            bar = 1e-11 if c == 2 else 1e-12
+            assert rel(osim.e3[c, j], rsim.e3[c, j]) < 1e-12, (j, "e3", c)
+            assert rel(osim.b3[c, j], rsim.b3[c, j]) < bar, (j, "b3", c)
+        for s, r in zip(osim.species, rsim.species):
+            assert np.array_equal(s.kpic, r.kpic), j       # per-tile counts, bit-exact
+            rp = r.particles()
+            # every particle's cell, as a per-cell count (bit-exact)
+            ncell = (grid.nx + 1) * (grid.ny + 1)
+            assert np.array_equal(
+                np.bincount(qpo.cell_of(grid.nx, grid.ny, s.part), minlength=ncell),
+                np.bincount(qpo.cell_of(grid.nx, grid.ny, rp), minlength=ncell)), j
+            k0 = 0
+            for k, n in enumerate(s.kpic):
+                a = s.part[k0:k0 + n]
+                b = rp[k0:k0 + n]
+                # same particles tile by tile (the reference's order inside a
+                # tile is its hole-fill order): compare sorted by cell, then x, y
+                ka = np.lexsort((a[:, 1], a[:, 0]))
+                kb = np.lexsort((b[:, 1], b[:, 0]))
+                assert np.abs(a[ka] - b[kb]).max(initial=0.0) < 1e-12, (j, k)
+                k0 += n
+
+
+def test_composed_slices_match_the_reference(ref):
+    """qpo.Simulation's slice sequence (simulation_class.f03:643-736) against the
+    same loop composed independently over the reference's own kernels
+    (oracle/refslice.py): 8 slices of a 64^2 grid through the beam."""
+    beam = qpo.BeamSpec(center=(0.0, 0.0, 1.0), sigma=(1.0, 1.0, 0.6), np_=(16, 16, 32),
+                        peak=4.0)
+    _composed(6, 6, [qpo.SpeciesSpec(ppc=(2, 2))], [beam], 8)
+
+
+def test_composed_slices_nonsquare_two_species_two_beams(ref):
+    """The same on a 64 x 32 grid with an electron and a heavy positive species
+    of different ppc and density, and two beams (the per-species additions of
+    species2d_class.f03:166-218 and the beam copy-adds of :652-655)."""
+    b1 = qpo.BeamSpec(center=(0.0, 0.0, 1.0), sigma=(1.0, 1.0, 0.6), np_=(16, 16, 32), peak=4.0)
+    b2 = qpo.BeamSpec(center=(1.0, -0.5, 1.5), sigma=(0.8, 1.2, 0.5), np_=(8, 16, 32), peak=2.0)
+    sp = [qpo.SpeciesSpec(ppc=(2, 2)), qpo.SpeciesSpec(ppc=(1, 3), density=0.5, q=1.0, m=100.0)]
+    _composed(6, 5, sp, [b1, b2], 8)
