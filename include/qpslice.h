@@ -93,6 +93,9 @@ enum { QP_PROF_FFT = 0, QP_PROF_TRANSPOSE, QP_PROF_KSPACE, QP_PROF_ELEMENTWISE,
        QP_PROF_SORT, QP_PROF_BORDER, QP_PROF_BEAM_DEPOSIT, QP_PROF_BEAM_PUSH,
        QP_PROF_COUNT };
 int qp_ctx_profile(qp_ctx *ctx, int enable);   /* enabling also resets */
+/* the same for the classes in mask only (bit k = class k; 0 = off), so a
+ * timed run can carry events on one kernel class at little cost */
+int qp_ctx_profile_classes(qp_ctx *ctx, unsigned mask);
 /* bytes: the algorithmic bytes of those launches (operands read or written
  * once: particle records and field planes), the figure a roofline uses */
 int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,

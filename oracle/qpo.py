@@ -569,7 +569,10 @@ class Simulation:
     simulation_class.f03:611-765; one slice restates :643-736."""
 
     def __init__(self, grid: Grid, species: list, beams: list, dt: float,
-                 iter_: int = 2, keep_q3=False):
+                 iter_: int = 2, keep_q3=False, beam_parts=None):
+        """beam_parts: optional list of (np, 7) particle arrays that replace
+        the loader's beams (a large beam loaded elsewhere by a loader that is
+        itself held bit-exact to beam_init)."""
         self.grid = grid
         self.dex = grid.dx
         self.dxi = grid.dz
@@ -580,9 +583,12 @@ class Simulation:
         self.beam_specs = beams
         self.beams = []
         rng = rng_new()
-        for b in beams:
-            part, qm, ierr = beam_init(grid, b, rng=rng)
-            assert ierr == 0
+        for i, b in enumerate(beams):
+            if beam_parts is not None:
+                part = np.ascontiguousarray(beam_parts[i], dtype=np.float64)
+            else:
+                part, qm, ierr = beam_init(grid, b, rng=rng)
+                assert ierr == 0
             self.beams.append([part, b.q / b.m])
         nx, ny, nz = grid.nx, grid.ny, grid.nz
         self.e3 = np.zeros((3, nz + 1, ny + 1, nx + 1))

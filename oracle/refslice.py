@@ -44,14 +44,16 @@ class RefSpecies:
         nx, ny = self.nx, self.ny
         xppc, yppc = self.ppc
         qm = self.den * self.qsign / abs(self.qsign) / float(xppc) / float(yppc)
-        rows = []
-        for i in range(2, nx):
-            for j in range(2, ny + 1):
-                for ix in range(xppc):
-                    for iy in range(yppc):
-                        rows.append(((ix + 0.5) / xppc + i - 1, (iy + 0.5) / yppc + j - 1,
-                                     0.0, 0.0, 0.0, 1.0, 1.0, qm))
-        return np.array(rows, dtype=np.float64)
+        # the loop nest i, j, ix, iy (iy fastest), evaluated left to right
+        i, j, ix, iy = np.meshgrid(np.arange(2, nx), np.arange(2, ny + 1), np.arange(xppc),
+                                   np.arange(yppc), indexing="ij")
+        n = i.size
+        part = np.zeros((n, 8))
+        part[:, 0] = ((ix.ravel() + 0.5) / xppc + i.ravel()) - 1
+        part[:, 1] = ((iy.ravel() + 0.5) / yppc + j.ravel()) - 1
+        part[:, 5] = part[:, 6] = 1.0
+        part[:, 7] = qm
+        return part
 
     def renew(self):
         """renew_species2d :134-164: new lattice, qn = -(deposit, folded)."""

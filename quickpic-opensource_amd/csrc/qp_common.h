@@ -107,7 +107,7 @@ struct qp_ctx {
     qp::DevBuf red;               // reduction partials (double)
     size_t max_lds;               // LDS bytes one workgroup may use
     // kernel-class timing (qp_ctx_profile)
-    int prof_on;
+    unsigned prof_on;     // bit k: launches of class k are timed
     int prof_tag;         // tag of the launches being enqueued (prof_discard)
     std::vector<qp::ProfEv> prof_pending;
     std::vector<hipEvent_t> prof_pool;
@@ -259,6 +259,9 @@ int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
 int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz, qp_field2d *psi,
                       qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd);
 
+// the value a new context gives tuning knob `knob` (qp_ctx_tune)
+int tune_default_of(int precision, int knob);
+
 // ---- kernel-class timing ----
 int prof_begin(qp_ctx *c, int kind, double bytes);   // returns a slot or -1
 void prof_end(qp_ctx *c, int slot);
@@ -272,7 +275,7 @@ struct ProfScope {
     qp_ctx *c;
     int slot;
     ProfScope(qp_ctx *cc, int kind, double bytes = 0.0)
-        : c(cc), slot(cc->prof_on ? prof_begin(cc, kind, bytes) : -1) {}
+        : c(cc), slot((cc->prof_on >> kind) & 1u ? prof_begin(cc, kind, bytes) : -1) {}
     ~ProfScope() { if (slot >= 0) prof_end(c, slot); }
 };
 

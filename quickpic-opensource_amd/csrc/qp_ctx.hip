@@ -186,11 +186,15 @@ using namespace qp;
 static const int TUNE_VALUES[QP_TUNE_COUNT] = {4, 11, 2, 3, 5, 2, 2};
 static const int TUNE_DEFAULT[QP_TUNE_COUNT] = {0, 0, 1, 0, 0, 0, 0};
 
-static int tune_default(const qp_ctx *c, int knob)
+namespace qp {
+int tune_default_of(int precision, int knob)
 {
-    if (knob == QP_TUNE_FFT && c->prec == QP_FP64) return 1;
+    if (knob == QP_TUNE_FFT && precision == QP_FP64) return 1;
     return TUNE_DEFAULT[knob];
 }
+}  // namespace qp
+
+static int tune_default(const qp_ctx *c, int knob) { return tune_default_of(c->prec, knob); }
 
 extern "C" {
 
@@ -268,9 +272,9 @@ int qp_ctx_destroy(qp_ctx *c)
     return QP_OK;
 }
 
-int qp_ctx_profile(qp_ctx *c, int enable)
+int qp_ctx_profile_classes(qp_ctx *c, unsigned mask)
 {
-    QP_REQUIRE(c, QP_ERR_ARG, "qp_ctx_profile: null context");
+    QP_REQUIRE(c, QP_ERR_ARG, "qp_ctx_profile_classes: null context");
     int rc = prof_resolve(c);
     if (rc) return rc;
     for (int k = 0; k < QP_PROF_COUNT; ++k) {
@@ -278,8 +282,13 @@ int qp_ctx_profile(qp_ctx *c, int enable)
         c->prof_n[k] = 0;
         c->prof_bytes[k] = 0.0;
     }
-    c->prof_on = enable ? 1 : 0;
+    c->prof_on = mask & ((1u << QP_PROF_COUNT) - 1u);
     return QP_OK;
+}
+
+int qp_ctx_profile(qp_ctx *c, int enable)
+{
+    return qp_ctx_profile_classes(c, enable ? ~0u : 0u);
 }
 
 int qp_ctx_profile_get(qp_ctx *c, int kind, double *total_ms, int64_t *count, double *bytes)

@@ -438,48 +438,46 @@ static int gather_comps(qp_field2d *const *fs, const int *kinds, int nf, R **rs,
     return nc;
 }
 
+// real -> k of nc components: rs[i] (real planes, pitch ldr) to ks[i]
+// (k planes, pitch ldk); sr: nc scratch planes of (ny+1) x ldr for the x
+// pass; mx, my: per-component sine/cosine bit masks; W: twiddles of 2nx and
+// 2ny points
 template <typename R>
-static int r2k_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
+static int r2k_core(qp_ctx *c, R *const *rs, R *const *ks, int nc, int mx, int my, R *sr,
+                    int ldr, int ldk, const cplx<R> *twx, const cplx<R> *twy)
 {
-    R *rs[MAX_BATCH], *ks[MAX_BATCH];
-    int mx, my;
-    const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
-    if (nc <= 0) return nc;
-    R *sr = c->scr_r.as<R>();
+    const size_t pl = (size_t)(c->ny + 1) * ldr;
     CompSet<R> a{}, b{}, y{};
     for (int i = 0; i < nc; ++i) {
         a.src[i] = rs[i];
-        a.dst[i] = sr + i * c->rplane;
+        a.dst[i] = sr + i * pl;
         b.src[i] = a.dst[i];
         b.dst[i] = ks[i];
         y.src[i] = ks[i];
         y.dst[i] = ks[i];
     }
     const double ani = 1.0 / ((double)c->nx * (double)c->ny);
-    int rc = rowfft0<R>(c, a, nc, c->ny + 1, c->nx, c->ldr, c->ldr, mx, ani,
-                          c->twx.as<cplx<R>>(), -1);
+    int rc = rowfft0<R>(c, a, nc, c->ny + 1, c->nx, ldr, ldr, mx, ani, twx, -1);
     if (rc) return rc;
     if (c->tune[QP_TUNE_FFT] == 1)   // y pass straight from the column pairs of the x pass
-        return rowfft<R, 1, pad_shift<R>()>(c, b, nc, c->nx + 1, c->ny, c->ldr, c->ldk, my, 1.0,
-                            c->twy.as<cplx<R>>(), -1);
-    rc = transpose<R>(c, b, nc, c->ldr, c->ldk, c->ny + 1, c->nx + 1);
-    if (!rc) rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 1.0,
-                               c->twy.as<cplx<R>>(), -1);
+        return rowfft<R, 1, pad_shift<R>()>(c, b, nc, c->nx + 1, c->ny, ldr, ldk, my, 1.0,
+                                            twy, -1);
+    rc = transpose<R>(c, b, nc, ldr, ldk, c->ny + 1, c->nx + 1);
+    if (!rc) rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, ldk, ldk, my, 1.0, twy, -1);
     return rc;
 }
 
+// k -> real of nc components, ks[i] to rs[i]; sk: nc scratch planes of
+// (nx+1) x ldk for the y pass (unused by the column-pair form)
 template <typename R>
-static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
+static int k2r_core(qp_ctx *c, R *const *rs, R *const *ks, int nc, int mx, int my, R *sk,
+                    int ldr, int ldk, const cplx<R> *twx, const cplx<R> *twy)
 {
-    R *rs[MAX_BATCH], *ks[MAX_BATCH];
-    int mx, my;
-    const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
-    if (nc <= 0) return nc;
-    R *sk = c->scr_k.as<R>();
+    const size_t pl = (size_t)(c->nx + 1) * ldk;
     CompSet<R> y{}, b{}, a{}, yc{};
     for (int i = 0; i < nc; ++i) {
         y.src[i] = ks[i];
-        y.dst[i] = sk + i * c->kplane;
+        y.dst[i] = sk + i * pl;
         b.src[i] = y.dst[i];
         b.dst[i] = rs[i];
         a.src[i] = rs[i];
@@ -489,17 +487,37 @@ static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
     }
     int rc;
     if (c->tune[QP_TUNE_FFT] == 1) {   // y pass written as column pairs of rs
-        rc = rowfft<R, 2, pad_shift<R>()>(c, yc, nc, c->nx + 1, c->ny, c->ldk, c->ldr, my, 2.0,
-                          c->twy.as<cplx<R>>(), -1);
+        rc = rowfft<R, 2, pad_shift<R>()>(c, yc, nc, c->nx + 1, c->ny, ldk, ldr, my, 2.0, twy,
+                                          -1);
     } else {
-        rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, c->ldk, c->ldk, my, 2.0,
-                          c->twy.as<cplx<R>>(), -1);
-        if (!rc) rc = transpose<R>(c, b, nc, c->ldk, c->ldr, c->nx + 1, c->ny + 1);
+        rc = rowfft0<R>(c, y, nc, c->nx + 1, c->ny, ldk, ldk, my, 2.0, twy, -1);
+        if (!rc) rc = transpose<R>(c, b, nc, ldk, ldr, c->nx + 1, c->ny + 1);
     }
     // rows 0..ny-1; row ny receives row 0 (guard copy, PPNCGUARD2L nvp = 1)
-    if (!rc) rc = rowfft0<R>(c, a, nc, c->ny, c->nx, c->ldr, c->ldr, mx, 2.0,
-                               c->twx.as<cplx<R>>(), c->ny);
+    if (!rc) rc = rowfft0<R>(c, a, nc, c->ny, c->nx, ldr, ldr, mx, 2.0, twx, c->ny);
     return rc;
+}
+
+template <typename R>
+static int r2k_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
+{
+    R *rs[MAX_BATCH], *ks[MAX_BATCH];
+    int mx, my;
+    const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
+    if (nc <= 0) return nc;
+    return r2k_core<R>(c, rs, ks, nc, mx, my, c->scr_r.as<R>(), c->ldr, c->ldk,
+                       c->twx.as<cplx<R>>(), c->twy.as<cplx<R>>());
+}
+
+template <typename R>
+static int k2r_t(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)
+{
+    R *rs[MAX_BATCH], *ks[MAX_BATCH];
+    int mx, my;
+    const int nc = gather_comps<R>(fs, kinds, nf, rs, ks, &mx, &my);
+    if (nc <= 0) return nc;
+    return k2r_core<R>(c, rs, ks, nc, mx, my, c->scr_k.as<R>(), c->ldr, c->ldk,
+                       c->twx.as<cplx<R>>(), c->twy.as<cplx<R>>());
 }
 
 int fft_r2k_batch(qp_ctx *c, qp_field2d *const *fs, const int *kinds, int nf)

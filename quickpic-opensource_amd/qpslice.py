@@ -131,9 +131,14 @@ class Ctx:
         _call(lib().qp_ctx_stream, self.h, C.byref(h))
         return h.value
 
-    def profile(self, enable=True):
-        """Starts (and resets) or stops kernel-class timing with HIP events."""
-        _call(lib().qp_ctx_profile, self.h, 1 if enable else 0)
+    def profile(self, enable=True, classes=None):
+        """Starts (and resets) or stops kernel-class timing with HIP events;
+        classes: the PROF_* kinds to time (default: all)."""
+        if classes is None:
+            _call(lib().qp_ctx_profile, self.h, 1 if enable else 0)
+        else:
+            mask = sum(1 << k for k in classes) if enable else 0
+            _call(lib().qp_ctx_profile_classes, self.h, C.c_uint(mask))
 
     def profile_get(self, kind):
         """(total milliseconds, launches, algorithmic bytes) of one kernel

@@ -288,3 +288,40 @@ def test_species_charge_diagnostic_cbq():
         gsim.slice(j)
         assert rel(q3.plane(1, j + 1), osim.q3s[0][j]) < 1e-10, j
     ctx.close()
+
+
+def test_slices_fp64_nonsquare_two_species_two_beams():
+    """64 x 32 grid, an electron species and a heavy positive one of other
+    ppc and density, two beams: FP64 slices against the oracle at 1e-10 with
+    bit-exact per-cell counts.  A swap of nx and ny in a transform or
+    k-space index, or a wrong order of the per-species additions
+    (species2d_class.f03:166-218) or of the beam copy-adds
+    (simulation_class.f03:652-655), fails here."""
+    banner("slices fp64 non-square, 2 species, 2 beams")
+    indx, indy, indz = 6, 5, 6
+    grid = qpo.Grid(indx, indy, indz)
+    b1 = qpo.BeamSpec(center=(0.0, 0.0, 1.0), sigma=(1.0, 1.0, 0.6), np_=(16, 16, 32), peak=4.0)
+    b2 = qpo.BeamSpec(center=(1.0, -0.5, 1.5), sigma=(0.8, 1.2, 0.5), np_=(8, 16, 32), peak=2.0)
+    specs = [qpo.SpeciesSpec(ppc=(2, 2)),
+             qpo.SpeciesSpec(ppc=(1, 3), density=0.5, q=1.0, m=100.0)]
+    osim = qpo.Simulation(grid, specs, [b1, b2], dt=10.0, iter_=2)
+    ctx = qps.Ctx(indx, indy, indz, box=BOX, precision=qps.FP64)
+    try:
+        sps = [qps.Species(ctx, ppc=s.ppc, density=s.density, q=s.q, m=s.m) for s in specs]
+        bms = [qps.Beam(ctx, center=b.center, sigma=b.sigma, sigma_v=b.sigma_v, np_=b.np_,
+                        gamma=b.gamma, peak=b.peak, dt=10.0) for b in (b1, b2)]
+        for bm, (part, _) in zip(bms, osim.beams):
+            assert np.array_equal(bm.download(), part)      # the loader shares the stream
+        gsim = qps.Sim(ctx, sps, bms, iter_=2)
+        osim.begin_step()
+        gsim.begin_step()
+        for j in range(1, 9):
+            osim.slice(j)
+            gsim.slice(j)
+            compare_state(osim, gsim, sps[0], j, 1e-10)
+            for sp, s in zip(sps, osim.species):
+                counts = np.bincount(qpo.cell_of(grid.nx, grid.ny, s.part),
+                                     minlength=grid.nx * grid.ny)
+                assert np.array_equal(sp.cell_counts(), counts), j
+    finally:
+        ctx.close()
