@@ -104,7 +104,7 @@ def run_leg(args, precision, peak, nwarm, nsteps, beam_part=None, breakdown=Fals
         ctx.tune(qps.TUNE_SPECULATE, args.speculate)
         t0 = time.time()
         sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
-        bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0), sigma=(0.3, 0.3, 0.5),
+        bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0), sigma=(args.sigma_r, args.sigma_r, 0.5),
                       sigma_v=(1.0, 1.0, 0.0),
                       np_=tuple(args.beam_np) if beam_part is None else (0, 0, 0),
                       gamma=2.0e4, peak=peak, dt=10.0,
@@ -355,6 +355,47 @@ def cpu_baseline(args, passes):
     return info
 
 
+def probe(args):
+    """One untimed 3-D step per precision and nb/n0 with the strict move
+    check: where the step stops (a particle crossing more than one tile in a
+    slice, PPPCHECK2L), and the corrector passes per slice."""
+    import numpy as np
+
+    import qpslice as qps
+    out = {}
+    for pk in [args.peak] + list(args.other_peaks):
+        for prec, name in ((qps.FP32, "fp32"), (qps.FP64, "fp64")):
+            ctx = qps.Ctx(args.ind, args.ind, args.indz, precision=prec)
+            try:
+                ctx.tune(qps.TUNE_STRICT, 1)
+                ctx.tune(qps.TUNE_SEED, args.fp32_seed if prec == qps.FP32 else 0)
+                sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
+                bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0),
+                              sigma=(args.sigma_r, args.sigma_r, 0.5), sigma_v=(1.0, 1.0, 0.0),
+                              np_=tuple(args.beam_np), gamma=2.0e4, peak=pk, dt=10.0)
+                sim = qps.Sim(ctx, [sp], [bm], iter_=2)
+                sim.begin_step()
+                passes, stop = [], None
+                t0 = time.time()
+                try:
+                    for j in range(1, ctx.nz + 1):
+                        passes.append(sim.slice(j, iter_max=args.iter_max, tol=args.tol)[0])
+                    sim.end_step()
+                except qps.QPError as e:
+                    stop = str(e)
+                p = np.array(passes)
+                out[f"nb/n0={pk:g} {name}"] = {
+                    "slices_done": len(passes), "stop": stop, "seconds": round(time.time() - t0, 2),
+                    "passes_mean": float(p.mean()) if len(p) else None,
+                    "passes_max": int(p.max()) if len(p) else None,
+                    "passes_per_128": [round(float(p[i:i + 128].mean()), 2)
+                                       for i in range(0, len(p), 128)]}
+                log(f"[bench] probe nb/n0={pk:g} {name}: {out[f'nb/n0={pk:g} {name}']}")
+            finally:
+                ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +423,10 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=2)
     ap.add_argument("--sweep", action="store_true",
                     help="also time the compiled launch shapes and corrector variants")
+    ap.add_argument("--sigma-r", type=float, default=0.3, help="driver's transverse sigma")
+    ap.add_argument("--probe", action="store_true",
+                    help="only run one strict-checked step per precision and nb/n0 and report "
+                         "where it stops")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB copy kernel (profiler counter calibration)")
     args = ap.parse_args()
@@ -391,6 +436,9 @@ def main():
         sys.exit(2)
     if args.calib_copy:
         calib_copy(args)
+        return
+    if args.probe:
+        probe(args)
         return
     import qpslice as qps
     legs = {}

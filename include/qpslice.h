@@ -133,9 +133,14 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * changes f = g - (the pass's input).  The stopping test and the result
  * of the converged pass are the plain iteration's; only the passes to reach
  * the tolerance change.  Default 0 (the reference's iteration).
+ * QP_TUNE_STRICT = 1 makes qp_slice_run wait for its slice's particle move
+ * check (PPPCHECK2L) and fail at that slice, as the reference stops there.
+ * Default 0: the check is read without waiting at a later slice's start or
+ * at the end of the step, and its error names the slice whose push failed
+ * (the slices after it are invalid).
  * value -1 restores a knob's default. */
 enum { QP_TUNE_DEPOSIT_Q = 0, QP_TUNE_AMJDP = 1, QP_TUNE_SPECULATE = 2, QP_TUNE_SEED = 3,
-       QP_TUNE_FFT = 4, QP_TUNE_PUSH = 5, QP_TUNE_MIX = 6, QP_TUNE_COUNT };
+       QP_TUNE_FFT = 4, QP_TUNE_PUSH = 5, QP_TUNE_MIX = 6, QP_TUNE_STRICT = 7, QP_TUNE_COUNT };
 int qp_ctx_tune(qp_ctx *ctx, int knob, int value);
 /* current value and the number of values of a knob */
 int qp_ctx_tune_get(const qp_ctx *ctx, int knob, int *value, int *nvalues);
@@ -220,6 +225,14 @@ int qp_species_create(qp_ctx *ctx, const qp_species_params *p, double dt,
                       double s, qp_species **out);
 int qp_species_destroy(qp_species *sp);
 int qp_species_renew(qp_species *sp, double s);                   /* :138 */
+/* the piecewise longitudinal density of profile 0 (fdist2d_class.f03:
+ * 190-201, 233-251): the charge of every (re)load at propagation distance s
+ * is scaled by fs interpolated at s between the n points s_pts (n = 0: a
+ * uniform plasma); reloads the plasma at s_now.  QP_ERR_ARG with the
+ * reference's messages when s lies outside the points or the points
+ * decrease. */
+int qp_species_set_longitudinal(qp_species *sp, int n, const double *s_pts,
+                                const double *fs, double s_now);
 int qp_species_qdp(qp_species *sp, qp_field2d *q);                /* :166 */
 int qp_species_extpsi(qp_species *sp, const qp_field2d *psi, double dex);
 int qp_species_amjdp(qp_species *sp, const qp_field2d *ef,
@@ -316,6 +329,13 @@ int qp_sim_field3d(qp_sim *sim, qp_field3d **bexyz, qp_field3d **bbxyz);
 /* optional target of the species charge diagnostic cbq (species index from
  * 0 in the order given to qp_sim_create); without one cbq still runs */
 int qp_sim_species_q3(qp_sim *sim, int species, qp_field3d *q3);
+/* optional 3-D diagnostics of the slice loop (simulation_class.f03:670-671,
+ * 696-698): which = QP_DIAG_PSI, psi (times dex^2) into plane j+1 of a
+ * 1-component field; QP_DIAG_CU, the last corrector pass's current (times
+ * dex) into a 3-component field; f3 = NULL stops it.  The reference keeps
+ * these when its input deck asks for psi or jx/jy/jz output. */
+enum { QP_DIAG_PSI = 0, QP_DIAG_CU = 1 };
+int qp_sim_diag3d(qp_sim *sim, int which, qp_field3d *f3);
 /* start of a 3-D step: beam deposits, zero 2-D fields (pipeline receive on
  * the first stage, ufield2d_class.f03:528-531), slice-1 copies (:639-641) */
 int qp_sim_begin_step(qp_sim *sim);
@@ -340,6 +360,11 @@ int qp_slice_finish(qp_sim *sim, int j, int iter_max, double tol, int *iters_use
 int qp_slice_residuals(const qp_sim *sim, double *res, int max, int *n);
 /* end of the 3-D step: beam push (:752-757) and species renew (:761-764) */
 int qp_sim_end_step(qp_sim *sim, double s);
+/* its two parts, for a caller that writes diagnostics in between, as the
+ * reference does (diag_simulation, :759): the beam push, then the renewal
+ * at propagation distance s */
+int qp_sim_push_beams(qp_sim *sim);
+int qp_sim_renew(qp_sim *sim, double s);
 
 #ifdef __cplusplus
 }

@@ -198,11 +198,29 @@ class TwissSpec:
 
 @dataclass
 class SpeciesSpec:
-    """Uniform plasma, profile 0 (fdist2d_class.f03:168-316)."""
+    """Uniform plasma, profile 0 (fdist2d_class.f03:168-316); piecewise: an
+    optional longitudinal profile (s points, fs factors) of :190-201."""
     ppc: tuple = (2, 2)
     density: float = 1.0
     q: float = -1.0
     m: float = 1.0
+    piecewise: tuple = None
+
+
+def density_factor(piecewise, s):
+    """den_temp of dist2d_000 (fdist2d_class.f03:233-251): fs interpolated at
+    s between the points s(i-1) <= s <= s(i); 1 without a profile."""
+    if piecewise is None:
+        return 1.0
+    ps, fs = piecewise
+    if s < ps[0] or s > ps[-1]:
+        raise ValueError("The s is out of the bound!")
+    for i in range(1, len(ps)):
+        if ps[i] < ps[i - 1]:
+            raise ValueError("s is not monotonically increasing!")
+        if s <= ps[i]:
+            return fs[i - 1] + (fs[i] - fs[i - 1]) / (ps[i] - ps[i - 1]) * (s - ps[i - 1])
+    return 1.0
 
 
 def beam_params(grid: Grid, b: BeamSpec) -> BeamParams:
@@ -506,11 +524,14 @@ class Species:
         self.cu, self.dcu, self.amu = z(3), z(2), z(3)
         self.renew()
 
-    def renew(self):
-        """init_species2d / renew_species2d: fresh particles, qn = -deposit."""
+    def renew(self, s=0.0):
+        """init_species2d / renew_species2d: fresh particles at propagation
+        distance s, qn = -deposit.  The density factor multiplies the
+        density first, as den_temp*den in dist2d_000 (:254)."""
         g = self.grid
-        self.part = plasma_init(g.nx, g.ny, self.spec.ppc[0], self.spec.ppc[1],
-                                self.spec.density, self.spec.q)
+        den = density_factor(self.spec.piecewise, s) * self.spec.density
+        self.part = plasma_init(g.nx, g.ny, self.spec.ppc[0], self.spec.ppc[1], den,
+                                self.spec.q)
         self.kpic = order_by_tile(g.nx, g.ny, self.part)
         qn = np.zeros((1, g.ny + 1, g.nx + 1))
         qdeposit(g.nx, g.ny, self.part, qn)
@@ -569,7 +590,7 @@ class Simulation:
     simulation_class.f03:611-765; one slice restates :643-736."""
 
     def __init__(self, grid: Grid, species: list, beams: list, dt: float,
-                 iter_: int = 2, keep_q3=False, beam_parts=None):
+                 iter_: int = 2, keep_q3=False, beam_parts=None, keep_diag=False):
         """beam_parts: optional list of (np, 7) particle arrays that replace
         the loader's beams (a large beam loaded elsewhere by a loader that is
         itself held bit-exact to beam_init)."""
@@ -595,8 +616,21 @@ class Simulation:
         self.b3 = np.zeros((3, nz + 1, ny + 1, nx + 1))
         self.q3 = np.zeros((1, nz + 1, ny + 1, nx + 1))
         self.q3s = [np.zeros((nz + 1, ny + 1, nx + 1)) for _ in species] if keep_q3 else None
-        self.psi3 = None
+        # psi3d and cu3d of the reference's diagnostics (simulation_class.f03:670-671, 696-698)
+        self.psi3 = np.zeros((nz + 1, ny + 1, nx + 1)) if keep_diag else None
+        self.cu3 = np.zeros((3, nz + 1, ny + 1, nx + 1)) if keep_diag else None
+        self.q3_initial()
         self.reset_fields()
+
+    def q3_initial(self):
+        """Plane 1 of each species' 3-D charge at a (re)load on the first
+        stage (species2d_class.f03:107-113, 151-157): the fresh plasma's
+        folded deposit, smoothed."""
+        if self.q3s is None:
+            return
+        for si, s in enumerate(self.species):
+            q = s.qn * -1.0
+            self.q3s[si][0] = k2r(1, smooth(self.tab, r2k(1, q)))[0]
 
     def reset_fields(self):
         """pipeline receive on stage 0 zeroes cu, epw, fxy, psit, bxyz
@@ -654,6 +688,8 @@ class Simulation:
             s.extpsi(psi, d)
         psi = psi * (d * d)
         self.psi = psi
+        if self.psi3 is not None:
+            self.psi3[j] = psi[0]
         dv = div(self.cu_k)
         psit_k = pot(tab, dv)
         psit_k = psit_k * -d
@@ -671,6 +707,8 @@ class Simulation:
             cu = cu * d
             amu = amu * d
             dcu = dcu * d
+            if l == iters and self.cu3 is not None:
+                self.cu3[:, j] = cu
             if l == iters:
                 for si, s in enumerate(self.species):
                     q3 = s.cbq(tab)
@@ -713,14 +751,17 @@ class Simulation:
             self.beams[i][0] = beam_push(part, self.e3, self.b3, qbm, self.dt3d,
                                          self.dex, self.dxi).copy()
 
-    def step(self, nslices=None):
+    def step(self, nslices=None, s_new=0.0):
+        """One 3-D step; the species are renewed at s_new (i*dt after step i,
+        simulation_class.f03:761-764)."""
         self.begin_step()
         n = self.grid.nz if nslices is None else nslices
         for j in range(1, n + 1):
             self.slice(j)
         self.push_beams()
         for s in self.species:
-            s.renew()
+            s.renew(s_new)
+        self.q3_initial()
 
 
 def _residual(new, old):

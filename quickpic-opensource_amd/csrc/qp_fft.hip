@@ -535,3 +535,4 @@ int fft_r2k(qp_field2d *f, int kind) { return fft_r2k_batch(f->ctx, &f, &kind, 1
 int fft_k2r(qp_field2d *f, int kind) { return fft_k2r_batch(f->ctx, &f, &kind, 1); }
 
 }  // namespace qp
+

@@ -41,6 +41,10 @@ struct qp_species {
     int *h_err;
     hipEvent_t err_ev;
     bool err_pending;
+    // piecewise longitudinal density (fdist2d_class.f03:233-251): points
+    // s and factors fs; empty for a uniform plasma
+    std::vector<double> lp_s, lp_fs;
+    double den_factor = 1.0;   // the factor at the last load
 };
 
 namespace qp {
@@ -1240,11 +1244,16 @@ __global__ void k_gather(int64_t np, Soa<R> a, Soa<R> b, const int *src)
 }
 
 // the move-check word into pinned host memory, in stream order (a kernel
-// store instead of a copy-engine transfer between the slice's kernels)
-__global__ void k_post_err(const int *err, int *h_err)
+// store instead of a copy-engine transfer between the slice's kernels):
+// h_err[0] = the word; h_err[1] = the slice of the first post that saw it
+// set (the word stays set until the host clears it, so that is the slice
+// whose push set it)
+__global__ void k_post_err(const int *err, int *h_err, int slice)
 {
     if (threadIdx.x == 0) {
-        *h_err = *err;
+        const int e = *err;
+        h_err[0] = e;
+        if (e && h_err[1] == 0) h_err[1] = slice;
         __threadfence_system();
     }
 }
@@ -1452,7 +1461,8 @@ static int distribute(qp_species *sp)
     qp_ctx *c = sp->ctx;
     const int nx = c->nx, ny = c->ny, xppc = sp->p.ppc[0], yppc = sp->p.ppc[1];
     const int64_t np = (int64_t)(nx - 2) * (ny - 1) * xppc * yppc;
-    const double qm = 1.0 * sp->p.density * sp->p.q / std::fabs(sp->p.q) / (double)xppc / (double)yppc;
+    const double qm = sp->den_factor * sp->p.density * sp->p.q / std::fabs(sp->p.q) / (double)xppc /
+                      (double)yppc;
     std::vector<R> hx((size_t)np), hy((size_t)np);
     int64_t n = 0;
     for (int i = 2; i <= nx - 1; ++i)
@@ -1476,6 +1486,30 @@ static int distribute(qp_species *sp)
     int rc = rebin_unsorted<R>(sp);
     if (rc) return rc;
     return set_q_uniform(sp, true, qm);
+}
+
+// The density factor at propagation distance s of dist2d_000
+// (fdist2d_class.f03:233-251): 1 for a uniform plasma; else the linear
+// interpolation of fs between the points s(i-1) <= s <= s(i), with the
+// reference's checks (s inside the points; the points increasing up to the
+// segment found)
+static int den_factor_at(qp_species *sp, double s)
+{
+    sp->den_factor = 1.0;
+    const size_t n = sp->lp_s.size();
+    if (n == 0) return QP_OK;
+    const std::vector<double> &ps = sp->lp_s, &fs = sp->lp_fs;
+    QP_REQUIRE(!(s < ps[0] || s > ps[n - 1]), QP_ERR_ARG,
+               "dist2d_000: The s is out of the bound! (s = %g)", s);
+    for (size_t i = 1; i < n; ++i) {
+        QP_REQUIRE(!(ps[i] < ps[i - 1]), QP_ERR_ARG,
+                   "dist2d_000: s is not monotonically increasing!");
+        if (s <= ps[i]) {
+            sp->den_factor = fs[i - 1] + (fs[i] - fs[i - 1]) / (ps[i] - ps[i - 1]) * (s - ps[i - 1]);
+            break;
+        }
+    }
+    return QP_OK;
 }
 
 // renew: particles and qn = -(folded deposit of the fresh plasma)
@@ -1508,8 +1542,8 @@ static int alloc_particles(qp_species *sp, int64_t np)
     if (!rc) rc = sp->off_new.alloc((size_t)(c->ncells + 1) * 4);
     if (!rc) rc = sp->err.alloc(sizeof(int));
     if (rc) return rc;
-    QP_HIP(hipHostMalloc((void **)&sp->h_err, sizeof(int), hipHostMallocDefault));
-    *sp->h_err = 0;
+    QP_HIP(hipHostMalloc((void **)&sp->h_err, 2 * sizeof(int), hipHostMallocDefault));
+    sp->h_err[0] = sp->h_err[1] = 0;
     QP_HIP(hipEventCreateWithFlags(&sp->err_ev, hipEventDisableTiming));
     sp->scan_bytes = 0;
     QP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sp->scan_bytes, sp->count.as<int>(),
@@ -1690,14 +1724,14 @@ int qp_species_create(qp_ctx *c, const qp_species_params *p, double dt, double s
     QP_REQUIRE(c && p && out, QP_ERR_ARG, "qp_species_create: null argument");
     QP_REQUIRE(p->ppc[0] >= 1 && p->ppc[1] >= 1 && p->q != 0.0 && p->m != 0.0, QP_ERR_ARG,
                "qp_species_create: bad species parameters");
-    (void)s;
     qp_species *sp = new qp_species();
     sp->ctx = c;
     sp->p = *p;
     sp->qbm = p->q / p->m;
     sp->dt = dt;
     const int64_t np = (int64_t)(c->nx - 2) * (c->ny - 1) * p->ppc[0] * p->ppc[1];
-    int rc = DISPATCH(c, alloc_particles, sp, np);
+    int rc = den_factor_at(sp, s);
+    if (!rc) rc = DISPATCH(c, alloc_particles, sp, np);
     if (!rc) rc = qp_field2d_create(c, 1, 0, &sp->q);
     if (!rc) rc = qp_field2d_create(c, 1, 0, &sp->qn);
     if (!rc) rc = qp_field2d_create(c, 3, 0, &sp->cu);
@@ -1729,8 +1763,20 @@ int qp_species_destroy(qp_species *sp)
 int qp_species_renew(qp_species *sp, double s)
 {
     SPCHECK(sp);
-    (void)s;
+    int rc = den_factor_at(sp, s);
+    if (rc) return rc;
     return DISPATCH(sp->ctx, renew_t, sp);
+}
+
+int qp_species_set_longitudinal(qp_species *sp, int n, const double *s, const double *fs,
+                                double s_now)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(n == 0 || (n >= 2 && s && fs), QP_ERR_ARG,
+               "qp_species_set_longitudinal: %d points (0, or at least 2)", n);
+    sp->lp_s.assign(s, s + n);
+    sp->lp_fs.assign(fs, fs + n);
+    return qp_species_renew(sp, s_now);
 }
 
 // qdp_species2d (species2d_class.f03:166-185): this%q = folded deposit;
@@ -1798,6 +1844,27 @@ int qp_species_cbq(qp_species *sp, qp_field3d *q3, int pos)
     if (!rc && q3) rc = qp_field2d_cb(sp->q, q3, pos, 1, d1, d1);
     return rc;
 }
+
+}  // extern "C"
+
+namespace qp {
+// the first plane of the species' 3-D charge at a (re)load on the first
+// stage (species2d_class.f03:107-113, 151-157): q = the fresh plasma's
+// folded deposit (-qn), smoothed, into plane 1
+int species_q3_initial(qp_species *sp, qp_field3d *q3)
+{
+    SPCHECK(sp);
+    const int d1[1] = {1};
+    double m1 = -1.0;
+    int rc = qp_field2d_mult(sp->q, sp->qn, 0, nullptr, nullptr, &m1);
+    if (!rc) rc = qp_field2d_fftrk(sp->q, 1);
+    if (!rc) rc = qp_field2d_smooth(sp->q, sp->q);
+    if (!rc) rc = qp_field2d_fftkr(sp->q, 1);
+    if (!rc && q3) rc = qp_field2d_cb(sp->q, q3, 1, 1, d1, d1);
+    return rc;
+}
+}  // namespace qp
+extern "C" {
 
 int qp_species_count(const qp_species *sp, int64_t *np)
 {
@@ -1916,29 +1983,38 @@ int species_check(qp_species *sp) { return check_err(sp, "pmove"); }
 // The slice loop does not wait for the move check of each push: the error
 // word is copied to pinned memory behind the push, and a later slice (or the
 // end of the step, which waits) reports it.
-int species_check_post(qp_species *sp)
+int species_check_post(qp_species *sp, int slice)
 {
     qp_ctx *c = sp->ctx;
-    hipLaunchKernelGGL(k_post_err, dim3(1), dim3(64), 0, c->stream, sp->err.as<int>(), sp->h_err);
+    hipLaunchKernelGGL(k_post_err, dim3(1), dim3(64), 0, c->stream, sp->err.as<int>(), sp->h_err,
+                       slice);
     QP_LAUNCH_CHECK();
     QP_HIP(hipEventRecord(sp->err_ev, c->stream));
     sp->err_pending = true;
     return QP_OK;
 }
 
+// The move check of the slices posted so far, once their push has run
+// (wait: block until it has).  The error names the slice whose push moved a
+// particle too far; the slices after it were computed from misplaced
+// particles and are invalid, like everything after the reference's stop.
 int species_check_poll(qp_species *sp, bool wait)
 {
     if (!sp->err_pending) return QP_OK;
     if (wait) QP_HIP(hipEventSynchronize(sp->err_ev));
     else if (hipEventQuery(sp->err_ev) != hipSuccess) return QP_OK;
     sp->err_pending = false;
-    const int h = *sp->h_err;
+    const int h = sp->h_err[0], slice = sp->h_err[1];
     if (!h) return QP_OK;
-    *sp->h_err = 0;
+    sp->h_err[0] = sp->h_err[1] = 0;
     QP_HIP(hipMemsetAsync(sp->err.p, 0, sizeof(int), sp->ctx->stream));
-    if (h >= (1 << 30)) set_error("pmove: particle position outside the grid");
-    else if (h >= (1 << 29)) set_error("pmove: the push's two passes disagree");
-    else set_error("pmove: particle left its tile by more than one tile (tile %d)", h);
+    if (h >= (1 << 30))
+        set_error("pmove: particle position outside the grid (push of slice %d)", slice);
+    else if (h >= (1 << 29))
+        set_error("pmove: the push's two passes disagree (push of slice %d)", slice);
+    else
+        set_error("pmove: particle left its tile by more than one tile (tile %d, push of "
+                  "slice %d)", h, slice);
     return QP_ERR_CHECK;
 }
 

@@ -14,10 +14,11 @@ namespace qp {
 int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                          const qp_field2d *psit, double dex);
 int species_check(qp_species *sp);
-int species_check_post(qp_species *sp);
+int species_check_post(qp_species *sp, int slice);
 int species_check_poll(qp_species *sp, bool wait);
 int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first);
 int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice);
+int species_q3_initial(qp_species *sp, qp_field3d *q3);
 int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
                         double dex, bool first, bool last, const qp_field2d *psi);
@@ -28,6 +29,8 @@ struct qp_sim {
     std::vector<qp_species *> sp;
     std::vector<qp_beam *> bm;
     std::vector<qp_field3d *> q3s;   // optional cbq targets, one per species
+    qp_field3d *psi3d = nullptr;     // optional diagnostics: psi (dim 1), current (dim 3)
+    qp_field3d *cu3d = nullptr;
     int iter;
     qp_field2d *f[QP_F_COUNT];
     qp_field3d *bexyz, *bbxyz;
@@ -103,6 +106,31 @@ int qp_sim_species_q3(qp_sim *s, int species, qp_field3d *q3)
     QP_REQUIRE(s && species >= 0 && species < (int)s->sp.size(), QP_ERR_ARG,
                "qp_sim_species_q3: bad species index");
     s->q3s[species] = q3;
+    // plane 1 holds the smoothed charge of the plasma as loaded
+    return q3 ? species_q3_initial(s->sp[species], q3) : QP_OK;
+}
+
+int qp_sim_diag3d(qp_sim *s, int which, qp_field3d *f3)
+{
+    QP_REQUIRE(s && (which == QP_DIAG_PSI || which == QP_DIAG_CU), QP_ERR_ARG,
+               "qp_sim_diag3d: bad argument");
+    QP_REQUIRE(!f3 || (f3->ctx == s->ctx && f3->dim == (which == QP_DIAG_PSI ? 1 : 3)),
+               QP_ERR_ARG, "qp_sim_diag3d: the 3-D field must have %d component(s)",
+               which == QP_DIAG_PSI ? 1 : 3);
+    (which == QP_DIAG_PSI ? s->psi3d : s->cu3d) = f3;
+    return QP_OK;
+}
+
+// dim planes of a 2-D field's real space into plane lpos of a 3-D field
+// (the copy of cb, field2d_class.f03:538-575, whatever the field's state)
+static int copy_planes(qp_ctx *c, const qp_field2d *f, qp_field3d *f3, int lpos)
+{
+    size_t e = c->esize, pl = c->rplane, st = (size_t)(c->nz + 1) * pl;
+    for (int i = 0; i < f3->dim; ++i) {
+        char *dst = (char *)f3->buf.p + ((size_t)i * st + (size_t)(lpos - 1) * pl) * e;
+        const char *src = (const char *)f->rs.p + (size_t)i * pl * e;
+        QP_HIP(hipMemcpyAsync(dst, src, pl * e, hipMemcpyDeviceToDevice, c->stream));
+    }
     return QP_OK;
 }
 
@@ -304,6 +332,9 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     // its one product is the 3-D charge, so it runs for species that have one
     for (size_t i = 0; i < s->sp.size(); ++i)
         if (s->q3s[i]) RC(qp_species_cbq(s->sp[i], s->q3s[i], j + 1));
+    // the current of the last pass, scaled by dex, into cu3d (:696-698):
+    // its real-space planes are kept by the forward transform
+    if (s->cu3d) RC(copy_planes(c, cu, s->cu3d, j + 1));
     // final E_perp and predictor current (:721-726); B is already assembled
     RC(fused_efield(fxy, bxyz));
     RC(fused_predict_cu(cu, dcu, dxi));
@@ -311,7 +342,12 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     for (qp_species *sp : s->sp) RC(species_push_nocheck(sp, fxy, bxyz, psit, dex));
     // slice j+1 of the 3-D wake (:731-735)
     RC(fused_slice_write(fxy, psit, bxyz, psi, s->bexyz, s->bbxyz, j + 1, dex, dd));
-    for (qp_species *sp : s->sp) RC(species_check_post(sp));
+    // psi * dex^2 into psi3d (:670-671); the slice write applied the scaling
+    if (s->psi3d) RC(copy_planes(c, psi, s->psi3d, j + 1));
+    for (qp_species *sp : s->sp) RC(species_check_post(sp, j));
+    // strict move check: wait for this slice's check before returning
+    if (c->tune[QP_TUNE_STRICT])
+        for (qp_species *sp : s->sp) RC(species_check_poll(sp, true));
     s->last_passes = l;
     s->last_adaptive = adaptive;
     if (iters_used) *iters_used = l;
@@ -337,15 +373,31 @@ int qp_slice_run(qp_sim *s, int j, int iter_max, double tol, int *iters_used, do
     return qp_slice_finish(s, j, iter_max, tol, iters_used, residual);
 }
 
-int qp_sim_end_step(qp_sim *s, double snew)
+int qp_sim_push_beams(qp_sim *s)
 {
-    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_end_step: null sim");
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_push_beams: null sim");
     qp_ctx *c = s->ctx;
     for (qp_species *sp : s->sp) RC(species_check_poll(sp, true));
     for (qp_beam *b : s->bm) RC(qp_beam_push(b, s->bexyz, s->bbxyz, c->dex, c->dxi));
-    for (qp_species *sp : s->sp) RC(qp_species_renew(sp, snew));
-    QP_HIP(hipStreamSynchronize(c->stream));
     return QP_OK;
+}
+
+int qp_sim_renew(qp_sim *s, double snew)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_renew: null sim");
+    for (size_t i = 0; i < s->sp.size(); ++i) {
+        RC(qp_species_renew(s->sp[i], snew));
+        if (s->q3s[i]) RC(species_q3_initial(s->sp[i], s->q3s[i]));
+    }
+    QP_HIP(hipStreamSynchronize(s->ctx->stream));
+    return QP_OK;
+}
+
+int qp_sim_end_step(qp_sim *s, double snew)
+{
+    QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_end_step: null sim");
+    RC(qp_sim_push_beams(s));
+    return qp_sim_renew(s, snew);
 }
 
 }  // extern "C"

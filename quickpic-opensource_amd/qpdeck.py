@@ -6,10 +6,12 @@ character), with the keys read in simulation_class.f03:151-224 (grid, box, time
 step, corrector passes, counts of beams and species) and 392-569 (per beam and
 per species).  The first `nbeams` beams and `nspecies` species are used, as in
 the reference.  This build implements beam profile 0 (bi-Gaussian,
-fdist3d_000) and species profile 0 (uniform, fdist2d_000); a deck that asks for
-another profile is rejected with the reference's message for it.  MPI layout
-(`nodes`), diagnostics (`diag`) and restart keys are read but have no effect
-on one GPU.
+fdist3d_000) and profile 2 (Twiss, fdist3d_002), and species profile 0
+(fdist2d_000) with a uniform or piecewise longitudinal density; a deck that
+asks for another profile is rejected with the reference's message for it.
+Diagnostics (`diag` of fields, species and beams) and beam restart files
+(`dump_restart`, `read_restart`) are written and read in the reference's HDF5
+format by qpdiag.py.  The MPI layout (`nodes`) has no effect on one GPU.
 """
 from __future__ import annotations
 
@@ -90,6 +92,7 @@ class SpeciesCfg:
     density: float = 1.0
     q: float = -1.0
     m: float = 1.0
+    piecewise: tuple = None     # (s points, density factors) or None
 
 
 @dataclass
@@ -153,11 +156,16 @@ def config(deck: dict) -> DeckCfg:
         if prof != 0:
             raise ValueError(f"Invalid species profile number: {prof} (this build supports 0)")
         lp = p.get("longitudinal_profile", "uniform")
-        if lp != "uniform":
-            raise ValueError(f"longitudinal profile '{lp}' is not supported (uniform only)")
+        pw = None
+        if lp == "piecewise":     # fdist2d_class.f03:197-201
+            pw = (tuple(float(x) for x in p["piecewise_s"]),
+                  tuple(float(x) for x in p["piecewise_density"]))
+        elif lp != "uniform":
+            raise ValueError(f"longitudinal profile '{lp}' is not supported "
+                             "(uniform, piecewise)")
         cfg.species.append(SpeciesCfg(ppc=tuple(int(x) for x in p["ppc"]),
                                       density=float(p["density"]), q=float(p["q"]),
-                                      m=float(p["m"])))
+                                      m=float(p["m"]), piecewise=pw))
     return cfg
 
 
@@ -165,12 +173,21 @@ class Run:
     """The deck's simulation on one GPU: context, species, beams and the
     slice driver (simulation_class.f03:611-765)."""
 
-    def __init__(self, cfg: DeckCfg, precision=qps.FP64, device=0, seed=0):
+    def __init__(self, cfg: DeckCfg, precision=qps.FP64, device=0, seed=0, deck=None,
+                 root="."):
         self.cfg = cfg
         self.ctx = qps.Ctx(cfg.indx, cfg.indy, cfg.indz, box=cfg.box, precision=precision,
                            device=device)
+        sim_keys = deck["simulation"] if deck else {}
+        # start3d (simulation_class.f03:193-201): a restart resumes after its step
+        self.start3d = (int(sim_keys["restart_timestep"]) + 1
+                        if bool(sim_keys.get("read_restart", False)) else 1)
+        s0 = (self.start3d - 1) * cfg.dt
         self.species = [qps.Species(self.ctx, ppc=s.ppc, density=s.density, q=s.q, m=s.m)
                         for s in cfg.species]
+        for sp, s in zip(self.species, cfg.species):
+            if s.piecewise is not None:
+                sp.set_longitudinal(s.piecewise[0], s.piecewise[1], s0)
         self.beams = []
         for i, b in enumerate(cfg.beams):
             if isinstance(b, TwissCfg):
@@ -187,19 +204,34 @@ class Run:
                                        npmax=b.npmax, centroid_x=b.centroid_x,
                                        centroid_y=b.centroid_y))
         self.sim = qps.Sim(self.ctx, self.species, self.beams, iter_=cfg.iter)
+        self.diag = None
+        if deck is not None:
+            import qpdiag
+            if self.start3d > 1:
+                qpdiag.read_restart(self, deck, root)
+            self.diag = qpdiag.Diagnostics(self, deck, root)
 
     def step(self, i, tol=0.0, iter_max=None):
-        """3-D step i (1-based): every xi slice, then the beam push and the
-        plasma renewal at s = i*dt (simulation_class.f03:611-765).  Returns
-        the corrector passes of each slice."""
+        """3-D step i (1-based): every xi slice, the beam push, the
+        diagnostics of step i, then the plasma renewal at s = i*dt
+        (simulation_class.f03:611-765).  Returns the corrector passes of each
+        slice."""
         sim, nz = self.sim, self.ctx.nz
         sim.begin_step()
         passes = [sim.slice(j, iter_max=iter_max, tol=tol)[0] for j in range(1, nz + 1)]
-        sim.end_step(i * self.cfg.dt)
+        sim.push_beams()
+        if self.diag is not None:
+            self.diag.dump(i)
+        sim.renew(i * self.cfg.dt)
         return passes
 
     def run(self, tol=0.0, iter_max=None):
-        return [self.step(i, tol, iter_max) for i in range(1, self.cfg.nstep3d + 1)]
+        """go_simulation: the initial dump at step 0 of a fresh run
+        (:606-609), then steps start3d..nstep3d."""
+        if self.diag is not None and self.start3d == 1:
+            self.diag.dump(0)
+        return [self.step(i, tol, iter_max)
+                for i in range(self.start3d, self.cfg.nstep3d + 1)]
 
     def close(self):
         self.ctx.close()

@@ -34,10 +34,11 @@ FP64, FP32 = 0, 1
  PROF_AMJDP, PROF_PUSH, PROF_SORT, PROF_BORDER, PROF_BEAM_DEPOSIT, PROF_BEAM_PUSH) = range(12)
 PROF_NAMES = ("fft", "transpose", "kspace", "elementwise", "deposit_q", "extpsi", "amjdp",
               "push", "sort", "border", "beam_deposit", "beam_push")
-TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE, TUNE_SEED, TUNE_FFT, TUNE_PUSH, TUNE_MIX = (
-    0, 1, 2, 3, 4, 5, 6)
+(TUNE_DEPOSIT_Q, TUNE_AMJDP, TUNE_SPECULATE, TUNE_SEED, TUNE_FFT, TUNE_PUSH, TUNE_MIX,
+ TUNE_STRICT) = (0, 1, 2, 3, 4, 5, 6, 7)
 (F_QB, F_QE, F_PSIT, F_PSI, F_DIVVPOT, F_FXY, F_BXYZ, F_CU, F_DCU, F_AMU,
  F_EPW, F_EPWB) = range(12)
+DIAG_PSI, DIAG_CU = 0, 1
 
 
 class QPError(RuntimeError):
@@ -355,6 +356,13 @@ class Species:
     def renew(self, s=0.0):
         _call(lib().qp_species_renew, self.h, _D(s))
 
+    def set_longitudinal(self, s_pts, fs, s_now=0.0):
+        """Piecewise longitudinal density (fdist2d_class.f03:190-201,
+        233-251); reloads the plasma at s_now.  Empty lists: uniform."""
+        n = len(s_pts)
+        _call(lib().qp_species_set_longitudinal, self.h, n, _dbls(list(s_pts) or [0.0]),
+              _dbls(list(fs) or [0.0]), _D(s_now))
+
     def qdp(self, q):
         _call(lib().qp_species_qdp, self.h, q.h)
 
@@ -531,6 +539,11 @@ class Sim:
     def species_q3(self, i, q3):
         _call(lib().qp_sim_species_q3, self.h, i, q3.h if q3 is not None else None)
 
+    def diag3d(self, which, f3):
+        """Keeps psi (DIAG_PSI, 1 component) or the current (DIAG_CU, 3) of
+        every slice in the 3-D field f3 (None stops it)."""
+        _call(lib().qp_sim_diag3d, self.h, which, f3.h if f3 is not None else None)
+
     def begin_step(self):
         _call(lib().qp_sim_begin_step, self.h)
 
@@ -561,6 +574,14 @@ class Sim:
 
     def end_step(self, s=0.0):
         _call(lib().qp_sim_end_step, self.h, _D(s))
+
+    def push_beams(self):
+        """The first part of end_step: the beam push."""
+        _call(lib().qp_sim_push_beams, self.h)
+
+    def renew(self, s=0.0):
+        """The second part of end_step: the plasma renewal at s."""
+        _call(lib().qp_sim_renew, self.h, _D(s))
 
     def step(self, nslices=None):
         self.begin_step()

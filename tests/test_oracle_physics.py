@@ -192,3 +192,31 @@ def test_plasma_loader_is_the_quiet_start_lattice(nx, ny, xppc, yppc, den, q):
     assert part.shape == want.shape
     key = lambda a: a[np.lexsort((a[:, 1], a[:, 0]))]   # noqa: E731
     assert np.array_equal(key(part), key(want))
+
+
+def test_piecewise_density_factor():
+    """den_temp of dist2d_000 (fdist2d_class.f03:233-251) at points where the
+    linear interpolation is known, and its two error cases."""
+    pw = ((0.0, 10.0, 20.0, 20.0, 40.0), (1.0, 3.0, 0.5, 2.0, 0.0))
+    for s, want in ((0.0, 1.0), (5.0, 2.0), (10.0, 3.0), (15.0, 1.75), (20.0, 0.5),
+                    (30.0, 1.0), (40.0, 0.0)):
+        assert abs(qpo.density_factor(pw, s) - want) < 1e-15, s
+    assert qpo.density_factor(None, 123.0) == 1.0
+    with pytest.raises(ValueError, match="out of the bound"):
+        qpo.density_factor(pw, 40.5)
+    with pytest.raises(ValueError, match="monotonically"):
+        qpo.density_factor(((0.0, 10.0, 5.0, 30.0), (1.0, 1.0, 1.0, 1.0)), 20.0)
+    # a decreasing pair after the segment found is not examined (the loop exits)
+    assert qpo.density_factor(((0.0, 10.0, 5.0), (1.0, 2.0, 3.0)), 5.0) == 1.5
+
+
+def test_piecewise_density_scales_the_charge():
+    """The plasma at s carries den_temp times the uniform plasma's charge."""
+    grid = qpo.Grid(5, 5, 3)
+    pw = ((0.0, 100.0), (0.5, 2.5))
+    sp = qpo.Species(grid, qpo.SpeciesSpec(ppc=(2, 2), piecewise=pw), 0.1)
+    uni = qpo.Species(grid, qpo.SpeciesSpec(ppc=(2, 2)), 0.1)
+    assert np.allclose(sp.part[:, 7], 0.5 * uni.part[:, 7], rtol=0, atol=1e-16)
+    sp.renew(50.0)
+    assert np.allclose(sp.part[:, 7], 1.5 * uni.part[:, 7], rtol=0, atol=1e-16)
+    assert np.allclose(sp.qn, 1.5 * uni.qn, rtol=1e-14, atol=0)
