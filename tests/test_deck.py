@@ -33,8 +33,22 @@ def test_unsupported_profile_is_rejected():
     with pytest.raises(ValueError, match="Invalid beam profile number: 3"):
         qpdeck.config(deck)
     deck = qpdeck.load(DECK)
-    deck["species"][0]["longitudinal_profile"] = "piecewise"
-    with pytest.raises(ValueError, match="piecewise"):
+    deck["species"][0]["longitudinal_profile"] = "gaussian"
+    with pytest.raises(ValueError, match="gaussian"):
+        qpdeck.config(deck)
+
+
+def test_piecewise_longitudinal_profile_is_read():
+    """fdist2d_class.f03:197-201: piecewise_s and piecewise_density."""
+    deck = qpdeck.load(DECK)
+    sp = deck["species"][0]
+    sp["longitudinal_profile"] = "piecewise"
+    sp["piecewise_s"] = [0, 5, 30]
+    sp["piecewise_density"] = [1, 0.5, 2]
+    cfg = qpdeck.config(deck)
+    assert cfg.species[0].piecewise == ((0.0, 5.0, 30.0), (1.0, 0.5, 2.0))
+    del sp["piecewise_s"]
+    with pytest.raises(KeyError):
         qpdeck.config(deck)
 
 
