@@ -366,6 +366,32 @@ int qp_sim_end_step(qp_sim *sim, double s);
 int qp_sim_push_beams(qp_sim *sim);
 int qp_sim_renew(qp_sim *sim, double s);
 
+/* ---------------------------------------------------------------------- */
+/* batched parameter scan (no counterpart in the reference, which runs one */
+/* simulation per MPI job)                                                 */
+/* ---------------------------------------------------------------------- */
+/* One HIP graph holding one xi slice of each of n simulations (their own
+ * contexts, so their own streams; any grids, species, beams and
+ * precisions; all on one device).  The slice is qp_slice_run's with these
+ * differences: the slice index is a device word of each simulation
+ * (qp_sim_begin_step sets it to 1, each captured slice advances it), so the
+ * one graph serves every slice; with tol > 0 all iter_max passes are
+ * enqueued and the kernels of the passes after the converged one return at
+ * once (the stopping test and the result are qp_slice_run's); the particle
+ * move check is read once per qp_scan_run; the mixed corrector and the 3-D
+ * diagnostics (cbq, psi, current) are not available (QP_ERR_UNSUPPORTED).
+ * Create after qp_sim_begin_step; the simulations must have profiling off. */
+typedef struct qp_scan qp_scan;
+int qp_scan_create(qp_sim *const *sims, int n, int iter_max, double tol, qp_scan **out);
+/* nslices slices of every simulation (each continues from its last slice),
+ * ordered after the work already enqueued on the simulations' streams and
+ * before later work; waits for them, then checks the particle moves */
+int qp_scan_run(qp_scan *scan, int nslices);
+/* passes[j - 1] = corrector passes of slice j (j = 1..nslices) of one
+ * simulation, as the captured slices recorded them */
+int qp_scan_passes(const qp_scan *scan, int instance, int *passes, int nslices);
+int qp_scan_destroy(qp_scan *scan);
+
 #ifdef __cplusplus
 }
 #endif

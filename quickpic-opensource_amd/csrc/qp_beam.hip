@@ -51,6 +51,19 @@ struct Ranorm {
         r4 = (int32_t)(mix() & 0x7FFFFFFF);
         r5 = (int32_t)((mix() & 0x7FFFFFFF) | 1);
     }
+    // the stream of a context (seed 0 continues it)
+    static Ranorm of(const qp_ctx *c)
+    {
+        Ranorm r(0);
+        r.r1 = c->rn_state[0]; r.r2 = c->rn_state[1]; r.r4 = c->rn_state[2];
+        r.r5 = c->rn_state[3]; r.iflg = c->rn_state[4]; r.r0 = c->rn_r0;
+        return r;
+    }
+    void save(qp_ctx *c) const
+    {
+        c->rn_state[0] = r1; c->rn_state[1] = r2; c->rn_state[2] = r4;
+        c->rn_state[3] = r5; c->rn_state[4] = iflg; c->rn_r0 = r0;
+    }
     double next()
     {
         const double h1l = 65531.0, h1u = 32767.0, h2l = 65525.0;
@@ -134,7 +147,7 @@ static int64_t load_beam(qp_ctx *c, const qp_beam_params &bp, uint64_t seed,
     const double brx = std::fmin(x0 + 5.0 * sigx, (double)(nx - 1));
     const double bry = std::fmin(y0 + 5.0 * sigy, (double)(ny - 1));
     const double brz = std::fmin(z0 + 5.0 * sigz, (double)(nz - 1));
-    Ranorm rng(seed);
+    Ranorm rng = seed == 0 ? Ranorm::of(c) : Ranorm(seed);
     int64_t npp = 0, npt = 1;
     int j = 0, k, l = bp.np[2];
     auto put = [&](double x, double y, double z, double px, double py, double pz) {
@@ -178,6 +191,7 @@ static int64_t load_beam(qp_ctx *c, const qp_beam_params &bp, uint64_t seed,
         }
         j = j + 1;
     }
+    if (seed == 0) rng.save(c);
     return npp;
 }
 
@@ -218,7 +232,7 @@ static int64_t load_beam_twiss(qp_ctx *c, const qp_beam_twiss_params &bp, uint64
     const double brx = std::fmin(x0 + 3.0 * sigx, (double)(nx - 1));
     const double bry = std::fmin(y0 + 3.0 * sigy, (double)(ny - 1));
     const double brz = std::fmin(z0 + 3.0 * sigz, (double)(nz - 1));
-    Ranorm rng(seed);
+    Ranorm rng = seed == 0 ? Ranorm::of(c) : Ranorm(seed);
     int64_t npp = 0, npt = 1;
     int j = 0, k, l = bp.np[2];
     auto put = [&](double x, double y, double z, double px, double py, double pz) {
@@ -262,6 +276,7 @@ static int64_t load_beam_twiss(qp_ctx *c, const qp_beam_twiss_params &bp, uint64
         }
         j = j + 1;
     }
+    if (seed == 0) rng.save(c);
     return npp;
 }
 
@@ -301,8 +316,16 @@ __global__ void k_beam_deposit(int64_t np, const double *__restrict__ x, const d
 // fixed-point charge
 // (assign: q2 is taken as zero, as after asc(0), so it is not read)
 template <typename R>
-__global__ void k_beam_slice_add(R *q2, const long long *q3, size_t n, int assign)
+__global__ void k_beam_slice_add(R *q2, const long long *q3, size_t n, int assign,
+                                 const int *jdev, int nz)
 {
+    // jdev: plane (slice - 1) = j read from device memory (captured slices),
+    // 1 <= j <= nz (the host checks it before a replay)
+    if (jdev) {
+        const int j = *jdev;
+        if (j < 1 || j > nz) return;
+        q3 += (size_t)j * n;
+    }
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const R v = (R)((double)q3[i] * (1.0 / BEAM_FIX));
@@ -544,24 +567,25 @@ int qp_beam_qdp(qp_beam *b)
     return QP_OK;
 }
 
-// qdpcopy_beam3d (beam3d_class.f03:136-153): q += beam q at plane `slice`
-static int slice_copy(qp_beam *b, qp_field2d *q, int slice, int assign)
+// qdpcopy_beam3d (beam3d_class.f03:136-153): q += beam q at plane `slice`,
+// or with jdev at plane j + 1, j read from device memory when the copy runs
+static int slice_copy(qp_beam *b, qp_field2d *q, int slice, int assign, const int *jdev = nullptr)
 {
     BCHECK(b);
     QP_REQUIRE(q && q->dim >= 1, QP_ERR_ARG, "qdpcopy: bad field");
     qp_ctx *c = b->ctx;
-    QP_REQUIRE(slice >= 1 && slice <= c->nz + 1, QP_ERR_ARG, "qdpcopy: slice %d", slice);
-    const long long *src = b->q3.as<long long>() + (size_t)(slice - 1) * c->rplane;
+    QP_REQUIRE(jdev || (slice >= 1 && slice <= c->nz + 1), QP_ERR_ARG, "qdpcopy: slice %d", slice);
+    const long long *src = b->q3.as<long long>() + (jdev ? 0 : (size_t)(slice - 1) * c->rplane);
     size_t n = c->rplane;
     dim3 grid((unsigned)((n + 255) / 256));
     {
     ProfScope _ps(c, QP_PROF_ELEMENTWISE, (assign ? 12.0 : 16.0) * n);
     if (c->prec == QP_FP64)
         hipLaunchKernelGGL(k_beam_slice_add<double>, grid, dim3(256), 0, c->stream,
-                           q->rs.as<double>(), src, n, assign);
+                           q->rs.as<double>(), src, n, assign, jdev, c->nz);
     else
         hipLaunchKernelGGL(k_beam_slice_add<float>, grid, dim3(256), 0, c->stream,
-                           q->rs.as<float>(), src, n, assign);
+                           q->rs.as<float>(), src, n, assign, jdev, c->nz);
     }
     QP_LAUNCH_CHECK();
     q->state = 0;
@@ -575,7 +599,15 @@ int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice) { return slice_copy(b,
 
 namespace qp {
 // the slice loop's first beam charge copy: qb = 0 + beam slice (asc then copyadd)
-int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice) { return slice_copy(b, q, slice, 1); }
+int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice, const int *jdev)
+{
+    return slice_copy(b, q, slice, 1, jdev);
+}
+// the copies of the other beams (qdpcopy), with the slice index on the device
+int beam_slice_add_dev(qp_beam *b, qp_field2d *q, const int *jdev)
+{
+    return slice_copy(b, q, 0, 0, jdev);
+}
 }  // namespace qp
 
 extern "C" {

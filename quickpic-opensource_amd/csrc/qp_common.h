@@ -35,6 +35,7 @@ constexpr int FLAG_WORDS = 64;          // converged flags (passes 0..63)
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
+
 #define QP_HIP(call)                                                        \
     do {                                                                    \
         hipError_t _e = (call);                                             \
@@ -124,6 +125,12 @@ struct qp_ctx {
     hipEvent_t pass_ev[2];
     const int *skip;      // non-null while a speculative pass is being enqueued
     int tune[QP_TUNE_COUNT];   // launch-shape variants (qp_ctx_tune)
+    // ranorm's state (part3d_lib77.f:1324-1383), which the reference keeps
+    // in SAVE'd DATA variables: every seed-0 beam loaded in this context
+    // draws from this one stream, beam after beam, as the reference's beams
+    // share it in one process
+    int32_t rn_state[5];  // r1, r2, r4, r5, iflg
+    double rn_r0;
 };
 
 struct qp_field2d {
@@ -246,9 +253,11 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
 // converged flag).  With qp the residual of pass l is written to h_res and,
 // when tol > 0 and it is below tol, the converged flag is set.
 // seed (predictor only): 1 records B_perp in ehist, 2 also extrapolates
-// epw <- 2 epw - ehist first (the corrector seed of QP_TUNE_SEED = 1)
+// epw <- 2 epw - ehist first (the corrector seed of QP_TUNE_SEED = 1); with
+// seed_hist (captured slices) the predictor reads the count of recorded
+// slices from device memory and extrapolates once it is 2 or more
 int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int seed = 0,
-               qp_field2d *ehist = nullptr);
+               qp_field2d *ehist = nullptr, const int *seed_hist = nullptr);
 // Anderson mixing (depth 1) of the corrector iterate after pass l (f.gprev,
 // f.fprev set; the pass was run by fused_pass with them): unless pass l has
 // converged, epw <- g_l - gamma (g_l - g_{l-1}) and B_perp reassembled; mix =
@@ -256,8 +265,12 @@ int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int 
 int fused_mix(const PassFields &f, double dex, int l, double tol, bool mix);
 int fused_predict_cu(qp_field2d *cu, qp_field2d *dcu, double dxi);
 int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz);
+// jdev (captured slices): plane lpos - 1 is read from device memory instead;
+// ctl (eager slices): the device slice words are set to lpos (the next
+// slice) and hist (the seed's recorded slices)
 int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz, qp_field2d *psi,
-                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd);
+                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd,
+                      const int *jdev = nullptr, int *ctl = nullptr, int hist = 0);
 
 // the value a new context gives tuning knob `knob` (qp_ctx_tune)
 int tune_default_of(int precision, int knob);

@@ -216,6 +216,10 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     c->prec = precision;
     for (int k = 0; k < QP_TUNE_COUNT; ++k) c->tune[k] = tune_default(c, k);
     c->device = device;
+    // ranorm's DATA state (part3d_lib77.f:1335-1337)
+    c->rn_state[0] = 885098780; c->rn_state[1] = 1824280461;
+    c->rn_state[2] = 1396483093; c->rn_state[3] = 55318673; c->rn_state[4] = 0;
+    c->rn_r0 = 0.0;
     c->esize = precision == QP_FP64 ? 8 : 4;
     std::memcpy(c->box, box, sizeof(c->box));
     // simulation_class.f03:174-188: dx = (max-min)/2^indx etc.

@@ -104,7 +104,13 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
            const cplx<R> *__restrict__ W, int guard_row, const int *skip, ResJob rj)
 {
     using S = FftShape<L, PS>;
-    if (skip && *skip) return;
+    if (skip && *skip) {
+        // a pass skipped after convergence (skip is the residual job's
+        // flag_in) still passes the converged flag on to the next pass
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && rj.nblk > 0 && rj.flag_out)
+            *rj.flag_out = 1;
+        return;
+    }
     if (blockIdx.x == 0 && blockIdx.y == 0) run_res_job(rj);
     constexpr int M = S::M, n = S::N, TF = S::TF;
     extern __shared__ unsigned char smem_raw[];

@@ -48,10 +48,13 @@ __global__ void __launch_bounds__(256)
 k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *__restrict__ cu,
        R *dcu, const R *__restrict__ amu, R *epw, const R *__restrict__ epwb, R *psit, R *bxyz,
        R aa, R idex, R dex, double *partial, int *flags, const int *skip, R *fprev, int hist,
-       double *mixp, R *ehist, int seed)
+       double *mixp, R *ehist, int seed, const int *seed_hist)
 {
     __shared__ double s_num[256], s_den[256], s_ma[MIX ? 256 : 1], s_mb[MIX ? 256 : 1];
     if (skip && *skip) return;
+    // captured slices: the slices recorded in ehist so far come from device
+    // memory (2 or more: extrapolate; fewer: only record)
+    if (seed && seed_hist) seed = *seed_hist >= 2 ? 2 : 1;
     if (!QP && blockIdx.x == 0 && threadIdx.x < FLAG_WORDS) flags[threadIdx.x] = 0;
     double num = 0.0, den = 0.0, ma = 0.0, mb = 0.0;
     const size_t cs = g.cs;
@@ -247,8 +250,19 @@ __global__ void k_efield(size_t n, size_t cs, R *fxy, const R *__restrict__ bxyz
 template <typename R>
 __global__ void k_slice_write(size_t n, size_t cs, R *fxy, const R *__restrict__ psit, R *bxyz,
                               R *psi, R *be1, R *be2, R *be3, R *bb1, R *bb2, R *bb3, R dex,
-                              R dd)
+                              R dd, const int *jdev, int nz, int *ctl, int jnext, int hist)
 {
+    // eager slices keep the device slice word of captured ones in step
+    if (ctl && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[0] = jnext;
+        ctl[1] = hist;
+    }
+    if (jdev) {   // plane lpos - 1 = j, read from device memory (captured slices)
+        const int j = *jdev;
+        if (j < 1 || j > nz) return;   // the host checks it before a replay
+        const size_t z = (size_t)j * n;
+        be1 += z; be2 += z; be3 += z; bb1 += z; bb2 += z; bb3 += z;
+    }
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x) {
         psi[e] = psi[e] * dd;
@@ -296,7 +310,7 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy)
 
 template <typename R>
 static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, int seed,
-                  qp_field2d *ehist)
+                  qp_field2d *ehist, const int *seed_hist)
 {
     qp_ctx *c = f.cu->ctx;
     double *part = c->red.as<double>();
@@ -312,7 +326,8 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, i
                        f.psit->ks.as<R>(), f.bxyz->ks.as<R>(), (R)(dex * dex), (R)(1.0 / dex),
                        (R)dex, part, c->flags.as<int>(), qp ? c->skip : nullptr,
                        mix ? f.fprev->ks.as<R>() : nullptr, l >= 2 ? 1 : 0,
-                       part + MIX_PARTIALS, seed ? ehist->ks.as<R>() : nullptr, qp ? 0 : seed);
+                       part + MIX_PARTIALS, seed ? ehist->ks.as<R>() : nullptr, qp ? 0 : seed,
+                       qp ? nullptr : seed_hist);
     }
     QP_LAUNCH_CHECK();
     if (qp) {
@@ -328,11 +343,12 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, i
 }
 
 int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int seed,
-               qp_field2d *ehist)
+               qp_field2d *ehist, const int *seed_hist)
 {
     QP_REQUIRE(!seed || (ehist && !qp), QP_ERR_ARG, "fused_pass: seed without history");
-    return f.cu->ctx->prec == QP_FP64 ? pass_t<double>(f, dex, qp, l, tol, seed, ehist)
-                                      : pass_t<float>(f, dex, qp, l, tol, seed, ehist);
+    return f.cu->ctx->prec == QP_FP64
+               ? pass_t<double>(f, dex, qp, l, tol, seed, ehist, seed_hist)
+               : pass_t<float>(f, dex, qp, l, tol, seed, ehist, seed_hist);
 }
 
 template <typename R>
@@ -402,27 +418,32 @@ int fused_efield(qp_field2d *fxy, const qp_field2d *bxyz)
 template <typename R>
 static int slice_write_t(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz,
                          qp_field2d *psi, qp_field3d *bexyz, qp_field3d *bbxyz, int lpos,
-                         double dex, double dd)
+                         double dex, double dd, const int *jdev, int *ctl, int hist)
 {
     qp_ctx *c = fxy->ctx;
-    const size_t n = c->rplane, st = (size_t)(c->nz + 1) * n, z = (size_t)(lpos - 1) * n;
+    const size_t n = c->rplane, st = (size_t)(c->nz + 1) * n;
+    const size_t z = jdev ? 0 : (size_t)(lpos - 1) * n;
     R *be = bexyz->buf.as<R>() + z, *bb = bbxyz->buf.as<R>() + z;
     {
     ProfScope _ps(c, QP_PROF_ELEMENTWISE, 18.0 * n * sizeof(R));
     hipLaunchKernelGGL(k_slice_write<R>, flat_grid(n), dim3(256), 0, c->stream, n, n,
                        fxy->rs.as<R>(), psit->rs.as<R>(), bxyz->rs.as<R>(), psi->rs.as<R>(), be,
-                       be + st, be + 2 * st, bb, bb + st, bb + 2 * st, (R)dex, (R)dd);
+                       be + st, be + 2 * st, bb, bb + st, bb + 2 * st, (R)dex, (R)dd, jdev,
+                       c->nz, ctl, lpos, hist);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
 }
 
 int fused_slice_write(qp_field2d *fxy, const qp_field2d *psit, qp_field2d *bxyz, qp_field2d *psi,
-                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd)
+                      qp_field3d *bexyz, qp_field3d *bbxyz, int lpos, double dex, double dd,
+                      const int *jdev, int *ctl, int hist)
 {
     return fxy->ctx->prec == QP_FP64
-               ? slice_write_t<double>(fxy, psit, bxyz, psi, bexyz, bbxyz, lpos, dex, dd)
-               : slice_write_t<float>(fxy, psit, bxyz, psi, bexyz, bbxyz, lpos, dex, dd);
+               ? slice_write_t<double>(fxy, psit, bxyz, psi, bexyz, bbxyz, lpos, dex, dd, jdev,
+                                       ctl, hist)
+               : slice_write_t<float>(fxy, psit, bxyz, psi, bexyz, bbxyz, lpos, dex, dd, jdev,
+                                      ctl, hist);
 }
 
 }  // namespace qp

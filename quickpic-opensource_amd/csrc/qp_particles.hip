@@ -329,6 +329,33 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R 
     emit_nodes(nacc, out, 0, g.ldr, halo, tile, tx, ty);
 }
 
+// Record words 6 and 7 (gamma and gamma - p_z, both 1 for a particle at
+// rest).  FP64 stores them as the reference does.  FP32 stores p6 - 1 and
+// p7 - 1: a float holding 1 + e keeps e only to 6e-8 absolute, so psi's
+// imprint on p7 (e = -qbm psi dex^2) and the p_z formed from p6 and p7 (a
+// difference of two numbers near 1/2, part2d_lib77.f:597-598) would carry
+// errors of 6e-8/|e| relative, 1e-4 where psi is 1e-3.  The deviations are
+// formed without that cancellation and are mathematically the same
+// quantities; downloads and uploads convert at the boundary.
+template <typename R> struct P67Dev {
+    static constexpr bool on = std::is_same<R, float>::value;
+};
+
+// WPGPSIPOST2L_QP (part2d_lib77.f:2207-2284): p7 = 1 + dx and
+// p6 = dx + (dx2 |v|^2 - dx^2 - 2 dx) / (2 (1 + dx)) + 1 for dx = -qbm psi dex^2;
+// in the deviation form p7 - 1 = dx and p6 - 1 = (dx2 |v|^2 + dx^2) / (2 (1 + dx))
+template <typename R>
+__device__ __forceinline__ void p67_from_psi(R dx, R dx2, R vx, R vy, R &p6, R &p7)
+{
+    if constexpr (P67Dev<R>::on) {
+        p7 = dx;
+        p6 = (dx2 * (vx * vx + vy * vy) + dx * dx) / (R(2) * (R(1) + dx));
+    } else {
+        p7 = R(1) + dx;
+        p6 = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
+    }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R dex)
@@ -344,8 +371,10 @@ k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R d
     R vx = s.px[i], vy = s.py[i];
     R dx = gather(f, amx, amy, dxp, dyp);
     dx = -(dx * dx2 * qbm);
-    s.p7[i] = R(1) + dx;
-    s.p6[i] = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
+    R p6, p7;
+    p67_from_psi(dx, dx2, vx, vy, p6, p7);
+    s.p7[i] = p7;
+    s.p6[i] = p6;
 }
 
 // WPGPSIPOST2L_QP for one particle of the cell at (fx, fy): p7 and p6 from
@@ -360,8 +389,7 @@ __device__ __forceinline__ void extpsi_one(const Corners<R> &f, R fx, R fy, R x,
     R amx = R(1) - dxp, amy = R(1) - dyp;
     R dx = gather(f, amx, amy, dxp, dyp);
     dx = -(dx * dx2 * qbm);
-    p7 = R(1) + dx;
-    p6 = dx + (dx2 * (vx * vx + vy * vy) - dx * dx - R(2) * dx) / (R(2) * (R(1) + dx)) + R(1);
+    p67_from_psi(dx, dx2, vx, vy, p6, p7);
 }
 
 // the six fields amjdp and push gather, at the four corners of one cell
@@ -413,11 +441,12 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
                                        R p6, R p7, R qm, R qtmh, R qbm, R dti, R dex, R idex,
                                        R (&acc)[8][4])
 {
+    constexpr bool DEV = P67Dev<R>::on;
     R dxp = x - fx, dyp = y - fy;
     R amx = R(1) - dxp, amy = R(1) - dyp;
-    R inv_part_7 = recip(p7);
+    R inv_part_7 = recip(DEV ? R(1) + p7 : p7);
     R qtmh1 = qtmh * inv_part_7;
-    R qtmh2 = qtmh1 * p6;
+    R qtmh2 = DEV ? qtmh1 + qtmh1 * p6 : qtmh1 * p6;
     R dx = cell_gather(F.ex, amx, amy, dxp, dyp);
     R dy = cell_gather(F.ey, amx, amy, dxp, dyp);
     R dz = cell_gather(F.ez, amx, amy, dxp, dyp);
@@ -440,8 +469,13 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
     dxp = qm1 * dxp;
     ox = R(0.5) * (v1 + vx);
     oy = R(0.5) * (v2 + vy);
-    oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
-    const R p6n = p7 + oz;
+    // p_z = (1 + |p_perp|^2)/(2 p7) - p7/2; with p7 = 1 + e7 it is
+    // (|p_perp|^2 - e7 (2 + e7)) / (2 p7), formed without the cancellation
+    if constexpr (DEV)
+        oz = R(0.5) * (dex * dex * (ox * ox + oy * oy) - p7 * (R(2) + p7)) * inv_part_7;
+    else
+        oz = R(0.5) * (R(1) + dex * dex * (ox * ox + oy * oy)) * inv_part_7 - R(0.5) * p7;
+    const R p6n = p7 + oz;   // gamma = (gamma - p_z) + p_z (both forms)
     oz = oz * idex;
     vx = (v1 - vx) * dti;
     vy = (v2 - vy) * dti;
@@ -1009,8 +1043,8 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
                 return Corners<R>{fnode[f][m], fnode[f][m + 1], fnode[f][m + 17], fnode[f][m + 18]};
             };
             const Corners<R> ex = lc(0), ey = lc(1), ez = lc(2), bx = lc(3), by = lc(4), bz = lc(5);
-            R qtmh1 = divq(qtmh, p7);
-            R qtmh2 = qtmh1 * p6;
+            R qtmh1 = divq(qtmh, P67Dev<R>::on ? R(1) + p7 : p7);
+            R qtmh2 = P67Dev<R>::on ? qtmh1 + qtmh1 * p6 : qtmh1 * p6;
             R dxp = x - (R)cx, dyp = y - (R)cy;
             R amx = R(1) - dxp, amy = R(1) - dyp;
             R dx = gather(ex, amx, amy, dxp, dyp);
@@ -1345,7 +1379,8 @@ static int rebin_unsorted(qp_species *sp)
 }
 
 template <typename R>
-static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
+static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold,
+                          R *halo = nullptr)
 {
     QP_REQUIRE(ncomp >= 1 && ncomp <= 8, QP_ERR_ARG, "deposit border: %d components", ncomp);
     TileGeom g = geom(c);
@@ -1353,7 +1388,8 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold)
     {
     ProfScope _ps(c, QP_PROF_BORDER, 2.0 * ncomp * (double)n * sizeof(R));
     hipLaunchKernelGGL(k_deposit_border<R>, dim3((n + 255) / 256, ncomp), dim3(256), 0, c->stream,
-                       g, c->halo.as<R>(), (size_t)c->ntiles * HALO, out, fold, c->skip);
+                       g, halo ? halo : c->halo.as<R>(), (size_t)c->ntiles * HALO, out, fold,
+                       c->skip);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -1401,7 +1437,7 @@ static int allow_lds(K kernel, size_t bytes, qp_ctx *c)
 }
 
 template <typename R, int V>
-static int deposit_q_v(qp_species *sp, const DepOut<R> &out)
+static int deposit_q_v(qp_species *sp, const DepOut<R> &out, R *halo)
 {
     qp_ctx *c = sp->ctx;
     constexpr DepShape d = DepTab<R>::q[V];
@@ -1415,30 +1451,32 @@ static int deposit_q_v(qp_species *sp, const DepOut<R> &out)
     const double words = sp->q_uniform ? 2.0 : 3.0;
     ProfScope _ps(c, QP_PROF_DEPOSIT_Q, (words * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(kern, dim3(c->ntiles), dim3(256), lds, c->stream, geom(c),
-                       soa<R>(sp->a), sp->off.as<int>(), out, c->halo.as<R>(), (R)sp->q_value);
+                       soa<R>(sp->a), sp->off.as<int>(), out, halo, (R)sp->q_value);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
 }
 
+// halo: the tile-border buffer (null: the context's)
 template <typename R>
-static int deposit_q_out(qp_species *sp, const DepOut<R> &out)
+static int deposit_q_out(qp_species *sp, const DepOut<R> &out, R *halo = nullptr)
 {
     int rc;
+    if (!halo) halo = sp->ctx->halo.as<R>();
     switch (sp->ctx->tune[QP_TUNE_DEPOSIT_Q]) {
-    case 1: rc = deposit_q_v<R, 1>(sp, out); break;
-    case 2: rc = deposit_q_v<R, 2>(sp, out); break;
-    case 3: rc = deposit_q_v<R, 3>(sp, out); break;
-    default: rc = deposit_q_v<R, 0>(sp, out); break;
+    case 1: rc = deposit_q_v<R, 1>(sp, out, halo); break;
+    case 2: rc = deposit_q_v<R, 2>(sp, out, halo); break;
+    case 3: rc = deposit_q_v<R, 3>(sp, out, halo); break;
+    default: rc = deposit_q_v<R, 0>(sp, out, halo); break;
     }
     if (rc) return rc;
-    return deposit_border<R>(sp->ctx, out, 1, 1);
+    return deposit_border<R>(sp->ctx, out, 1, 1, halo);
 }
 
 template <typename R>
-static int deposit_q_t(qp_species *sp, R *plane)
+static int deposit_q_t(qp_species *sp, R *plane, R *halo = nullptr)
 {
-    return deposit_q_out<R>(sp, dep_plain<R>(&plane, 1));
+    return deposit_q_out<R>(sp, dep_plain<R>(&plane, 1), halo);
 }
 
 // dist2d_000 (fdist2d_class.f03:212-316), one rank: x in [1,nx-1),
@@ -1478,7 +1516,9 @@ static int distribute(qp_species *sp)
     QP_HIP(hipMemcpyAsync(sp->a[0].p, hx.data(), bytes, hipMemcpyHostToDevice, c->stream));
     QP_HIP(hipMemcpyAsync(sp->a[1].p, hy.data(), bytes, hipMemcpyHostToDevice, c->stream));
     QP_HIP(hipStreamSynchronize(c->stream));
-    const double fillv[6] = {0.0, 0.0, 0.0, 1.0, 1.0, qm};
+    // p6 = p7 = 1 at rest (0 in the FP32 deviation form)
+    const double p67 = P67Dev<R>::on ? 0.0 : 1.0;
+    const double fillv[6] = {0.0, 0.0, 0.0, p67, p67, qm};
     for (int k = 2; k < 8; ++k) {
         int rc = launch_fill(c, sp->a[k].p, (size_t)np, fillv[k - 2]);
         if (rc) return rc;
@@ -1884,7 +1924,9 @@ static int download_t(const qp_species *sp, double *part, int64_t ld)
         QP_HIP(hipMemcpyAsync(h.data(), sp->a[k].p, h.size() * sizeof(R), hipMemcpyDeviceToHost,
                               c->stream));
         QP_HIP(hipStreamSynchronize(c->stream));
-        for (int64_t i = 0; i < sp->np; ++i) part[k + 8 * i] = (double)h[i];
+        // words 6 and 7 of an FP32 record are held as deviations from 1
+        const double base = (P67Dev<R>::on && (k == 5 || k == 6)) ? 1.0 : 0.0;
+        for (int64_t i = 0; i < sp->np; ++i) part[k + 8 * i] = (double)h[i] + base;
     }
     (void)ld;
     return QP_OK;
@@ -1908,7 +1950,8 @@ static int upload_t(qp_species *sp, const double *part, int64_t np)
     qp_ctx *c = sp->ctx;
     std::vector<R> h((size_t)np);
     for (int k = 0; k < 8; ++k) {
-        for (int64_t i = 0; i < np; ++i) h[i] = (R)part[k + 8 * i];
+        const double base = (P67Dev<R>::on && (k == 5 || k == 6)) ? 1.0 : 0.0;
+        for (int64_t i = 0; i < np; ++i) h[i] = (R)(part[k + 8 * i] - base);
         QP_HIP(hipMemcpyAsync(sp->a[k].p, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice,
                               c->stream));
         QP_HIP(hipStreamSynchronize(c->stream));
@@ -1979,6 +2022,17 @@ int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d 
     return DISPATCH(sp->ctx, push_t, sp, ef, bf, psit, dex, false);
 }
 int species_check(qp_species *sp) { return check_err(sp, "pmove"); }
+
+// The push leaves the particles in the alternate arrays and swaps the
+// host's buffer pointers (and, when it re-sorts in place, the cell offsets).
+// A captured slice repeats the device side only; these let its caller keep
+// the host side in step: which buffer holds the particles, and one swap.
+const void *species_buffer_key(const qp_species *sp) { return sp->a[0].p; }
+void species_flip(qp_species *sp, bool offsets)
+{
+    if (offsets) std::swap(sp->off, sp->off_new);
+    for (int k = 0; k < 8; ++k) std::swap(sp->a[k], sp->b[k]);
+}
 
 // The slice loop does not wait for the move check of each push: the error
 // word is copied to pinned memory behind the push, and a later slice (or the

@@ -6,6 +6,7 @@
 // context stream; the only host synchronisations are the corrector's stopping
 // test (when a tolerance is given) and the particle check at the slice end.
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "qp_common.h"
@@ -14,10 +15,13 @@ namespace qp {
 int species_push_nocheck(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                          const qp_field2d *psit, double dex);
 int species_check(qp_species *sp);
+const void *species_buffer_key(const qp_species *sp);
+void species_flip(qp_species *sp, bool offsets);
 int species_check_post(qp_species *sp, int slice);
 int species_check_poll(qp_species *sp, bool wait);
 int species_qdp_fused(qp_species *sp, qp_field2d *q, bool first);
-int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice);
+int beam_slice_assign(qp_beam *b, qp_field2d *q, int slice, const int *jdev = nullptr);
+int beam_slice_add_dev(qp_beam *b, qp_field2d *q, const int *jdev);
 int species_q3_initial(qp_species *sp, qp_field3d *q3);
 int species_amjdp_fused(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
                         const qp_field2d *psit, qp_field2d *cu, qp_field2d *dcu, qp_field2d *amu,
@@ -40,11 +44,50 @@ struct qp_sim {
     int prepared;           // slice whose first part has run (0: none)
     int last_passes;        // corrector passes of the last finished slice
     bool last_adaptive;     // whether its residuals were read back
+    // Captured slices (qp_scan): while `graph` is set, prepare and finish
+    // enqueue a slice whose index lives in device memory, so one captured
+    // slice serves every j.  ctl[0] = j (the slice to run next), ctl[1] =
+    // the slices recorded for the extrapolated seed; passes_dev[j] = the
+    // corrector passes slice j used.  begin_step resets them.
+    qp::DevBuf ctl, passes_dev;
+    bool graph = false;
 };
 
 using namespace qp;
 
 #define RC(x) do { int _r = (x); if (_r) return _r; } while (0)
+
+namespace {
+
+__global__ void k_ctl_reset(int *ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl[0] = 1;
+        ctl[1] = 0;
+    }
+}
+
+// End of a captured slice: the passes it used (pass l converged when flag
+// word l is set; without a tolerance all iter_max ran), then the next slice
+__global__ void k_slice_end(int *ctl, const int *flags, int iter_max, int adaptive, int seeding,
+                            int *passes, int nz)
+{
+    if (threadIdx.x != 0) return;
+    const int j = ctl[0];
+    if (j < 1 || j > nz) return;
+    int p = iter_max;
+    if (adaptive)
+        for (int l = 1; l < iter_max; ++l)
+            if (flags[l]) {
+                p = l;
+                break;
+            }
+    passes[j] = p;
+    ctl[0] = j + 1;
+    if (seeding) ctl[1] = ctl[1] + 1;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -68,6 +111,8 @@ int qp_sim_create(qp_ctx *c, qp_species *const *sp, int nsp, qp_beam *const *bm,
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->epw_hist);
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->gprev);
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->fprev);
+    if (!rc) rc = s->ctl.alloc(4 * sizeof(int));
+    if (!rc) rc = s->passes_dev.alloc((size_t)(c->nz + 2) * sizeof(int));
     if (rc) { qp_sim_destroy(s); return rc; }
     *out = s;
     return QP_OK;
@@ -82,6 +127,9 @@ int qp_sim_destroy(qp_sim *s)
     qp_field2d_destroy(s->epw_hist);
     qp_field2d_destroy(s->gprev);
     qp_field2d_destroy(s->fprev);
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    s->ctl.release();
+    s->passes_dev.release();
     delete s;
     return QP_OK;
 }
@@ -146,6 +194,8 @@ int qp_sim_begin_step(qp_sim *s)
     RC(launch_fill(c, epw->ks.p, epw->dim * c->kplane, 0.0));
     epw->state = 1;
     s->hist_slices = 0;
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, c->stream, s->ctl.as<int>());
+    QP_LAUNCH_CHECK();
     RC(qp_field2d_as(s->f[QP_F_FXY], 0.0));
     RC(qp_field2d_as(s->f[QP_F_PSIT], 0.0));
     RC(qp_field2d_as(s->f[QP_F_BXYZ], 0.0));
@@ -183,13 +233,18 @@ int qp_slice_prepare(qp_sim *s, int j)
     qp_field2d *qb = F[QP_F_QB], *qe = F[QP_F_QE], *psi = F[QP_F_PSI], *fxy = F[QP_F_FXY];
     qp_field2d *epwb = F[QP_F_EPWB];
     const int nsp = (int)s->sp.size();
-    // the move check of an earlier slice, if it has arrived
-    for (qp_species *sp : s->sp) RC(species_check_poll(sp, false));
+    // the move check of an earlier slice, if it has arrived (captured
+    // slices leave the error word on the device; qp_scan_run reads it)
+    if (!s->graph)
+        for (qp_species *sp : s->sp) RC(species_check_poll(sp, false));
     // beam charge (:650-655) and plasma charge (:657-661); the two forward
     // transforms are independent and run as one batch
+    const int *jdev = s->graph ? s->ctl.as<int>() : nullptr;
     if (s->bm.empty()) RC(qp_field2d_as(qb, 0.0));
-    for (size_t i = 0; i < s->bm.size(); ++i)
-        RC(i == 0 ? beam_slice_assign(s->bm[0], qb, j + 1) : qp_beam_qdpcopy(s->bm[i], qb, j + 1));
+    for (size_t i = 0; i < s->bm.size(); ++i) {
+        if (i == 0) RC(beam_slice_assign(s->bm[0], qb, j + 1, jdev));
+        else RC(jdev ? beam_slice_add_dev(s->bm[i], qb, jdev) : qp_beam_qdpcopy(s->bm[i], qb, j + 1));
+    }
     if (nsp == 0) RC(qp_field2d_as(qe, 0.0));
     for (int i = 0; i < nsp; ++i) RC(species_qdp_fused(s->sp[i], qe, i == 0));
     QP_REQUIRE(qb->state == 0 && qe->state == 0, QP_ERR_STATE, "fftrk: data is in ks");
@@ -235,8 +290,10 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     qp_field2d *fwd[3] = {cu, dcu, amu};
     const int fwd_kinds[3] = {1, 1, 3};
     const bool adaptive = tol > 0.0;
+    const bool graph = s->graph;
     // the mixed corrector (QP_TUNE_MIX) applies only to runs to a tolerance
-    const bool mixing = adaptive && c->tune[QP_TUNE_MIX] == 1;
+    // outside captured slices
+    const bool mixing = adaptive && !graph && c->tune[QP_TUNE_MIX] == 1;
     PassFields pc = pf;
     if (mixing) {
         pc.gprev = s->gprev;
@@ -248,7 +305,8 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         seed = s->hist_slices >= 2 ? 2 : 1;
         ++s->hist_slices;
     }
-    RC(fused_pass(pf, dex, false, 0, 0.0, seed, s->epw_hist));
+    RC(fused_pass(pf, dex, false, 0, 0.0, seed, s->epw_hist,
+                  graph && seed ? s->ctl.as<int>() + 1 : nullptr));
     RC(fft_k2r_batch(c, inv, inv_kinds, 2));
     psit->state = bxyz->state = 0;
     // corrector passes (:676-710)
@@ -290,6 +348,17 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     double res = -1.0;
     if (!adaptive) {
         for (l = 1; l <= iter_max; ++l) RC(pass(l));
+        l = iter_max;
+    } else if (graph) {
+        // every pass is enqueued; once pass l converges (flag word l), the
+        // kernels of the later passes return at once and pass the flag on
+        QP_REQUIRE(iter_max < FLAG_WORDS, QP_ERR_ARG, "captured slice: iter_max %d >= %d",
+                   iter_max, FLAG_WORDS);
+        c->skip = flags;
+        int rc = QP_OK;
+        for (l = 1; l <= iter_max && !rc; ++l) rc = pass(l);
+        c->skip = nullptr;
+        RC(rc);
         l = iter_max;
     } else if (nsp == 0 || !c->tune[QP_TUNE_SPECULATE] || iter_max >= FLAG_WORDS) {
         // read each residual in turn
@@ -341,7 +410,18 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     // plasma push (:727-730)
     for (qp_species *sp : s->sp) RC(species_push_nocheck(sp, fxy, bxyz, psit, dex));
     // slice j+1 of the 3-D wake (:731-735)
-    RC(fused_slice_write(fxy, psit, bxyz, psi, s->bexyz, s->bbxyz, j + 1, dex, dd));
+    RC(fused_slice_write(fxy, psit, bxyz, psi, s->bexyz, s->bbxyz, j + 1, dex, dd,
+                         graph ? s->ctl.as<int>() : nullptr, graph ? nullptr : s->ctl.as<int>(),
+                         s->hist_slices));
+    if (graph) {
+        hipLaunchKernelGGL(k_slice_end, dim3(1), dim3(64), 0, c->stream, s->ctl.as<int>(), flags,
+                           iter_max, adaptive ? 1 : 0, seed ? 1 : 0, s->passes_dev.as<int>(),
+                           c->nz);
+        QP_LAUNCH_CHECK();
+        if (iters_used) *iters_used = -1;
+        if (residual) *residual = -1.0;
+        return QP_OK;
+    }
     // psi * dex^2 into psi3d (:670-671); the slice write applied the scaling
     if (s->psi3d) RC(copy_planes(c, psi, s->psi3d, j + 1));
     for (qp_species *sp : s->sp) RC(species_check_post(sp, j));
@@ -398,6 +478,207 @@ int qp_sim_end_step(qp_sim *s, double snew)
     QP_REQUIRE(s, QP_ERR_ARG, "qp_sim_end_step: null sim");
     RC(qp_sim_push_beams(s));
     return qp_sim_renew(s, snew);
+}
+
+// ---- batched scan: one HIP graph holding one xi slice of many instances ----
+
+// The plasma push moves the particles between two sets of arrays and swaps
+// the host's pointers to them, which a graph cannot repeat: the scan holds
+// two graphs, one captured with each buffer parity, and launches the one
+// that matches where the particles are (the first species of the instances
+// tells), swapping the host pointers after each launch as the push would.
+struct qp_scan {
+    std::vector<qp_sim *> sims;
+    int iter_max = 0;
+    double tol = 0.0;
+    hipStream_t origin = nullptr;          // the stream the graphs are launched on
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<const void *> key0;        // per instance: particle buffer of parity 0
+    std::vector<char> offsets;             // per instance: the push swaps cell offsets
+};
+
+static void scan_flip(qp_scan *sc)
+{
+    for (size_t i = 0; i < sc->sims.size(); ++i)
+        for (qp_species *sp : sc->sims[i]->sp) species_flip(sp, sc->offsets[i] != 0);
+}
+
+int qp_scan_destroy(qp_scan *sc)
+{
+    if (!sc) return QP_OK;
+    if (sc->origin) (void)hipStreamSynchronize(sc->origin);
+    for (int p = 0; p < 2; ++p) {
+        if (sc->exec[p]) (void)hipGraphExecDestroy(sc->exec[p]);
+        if (sc->graph[p]) (void)hipGraphDestroy(sc->graph[p]);
+    }
+    for (hipEvent_t e : sc->join)
+        if (e) (void)hipEventDestroy(e);
+    if (sc->fork) (void)hipEventDestroy(sc->fork);
+    if (sc->origin) (void)hipStreamDestroy(sc->origin);
+    delete sc;
+    return QP_OK;
+}
+
+// The capture: the origin stream forks to every instance's own stream, each
+// instance enqueues one slice there (prepare + finish in graph mode, which
+// reads the slice index from its device word and ends with k_slice_end), and
+// the streams join back.  Instances are independent branches of the graph.
+static int scan_capture(qp_scan *sc, int parity)
+{
+    QP_HIP(hipStreamBeginCapture(sc->origin, hipStreamCaptureModeRelaxed));
+    int rc = QP_OK;
+    hipError_t e = hipEventRecord(sc->fork, sc->origin);
+    if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord", __FILE__, __LINE__);
+    for (size_t i = 0; i < sc->sims.size() && !rc; ++i) {
+        qp_sim *s = sc->sims[i];
+        qp_ctx *c = s->ctx;
+        e = hipStreamWaitEvent(c->stream, sc->fork, 0);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipStreamWaitEvent", __FILE__, __LINE__); break; }
+        const int hist = s->hist_slices;   // the captured seed counts on the device
+        s->graph = true;
+        rc = qp_slice_prepare(s, 1);
+        if (!rc) rc = qp_slice_finish(s, 1, sc->iter_max, sc->tol, nullptr, nullptr);
+        s->graph = false;
+        s->prepared = 0;
+        s->hist_slices = hist;
+        if (rc) break;
+        e = hipEventRecord(sc->join[i], c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(sc->origin, sc->join[i], 0);
+        if (e != hipSuccess) rc = hip_fail(e, "capture join", __FILE__, __LINE__);
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(sc->origin, &g);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    sc->graph[parity] = g;
+    QP_HIP(hipGraphInstantiate(&sc->exec[parity], g, nullptr, nullptr, 0));
+    return QP_OK;
+}
+
+int qp_scan_create(qp_sim *const *sims, int n, int iter_max, double tol, qp_scan **out)
+{
+    QP_REQUIRE(sims && out && n >= 1, QP_ERR_ARG, "qp_scan_create: bad argument");
+    QP_REQUIRE(iter_max >= 1 && iter_max < FLAG_WORDS, QP_ERR_ARG,
+               "qp_scan_create: iter_max %d outside 1..%d", iter_max, FLAG_WORDS - 1);
+    for (int i = 0; i < n; ++i) {
+        qp_sim *s = sims[i];
+        QP_REQUIRE(s && s->ctx, QP_ERR_ARG, "qp_scan_create: null simulation %d", i);
+        QP_REQUIRE(s->ctx->device == sims[0]->ctx->device, QP_ERR_ARG,
+                   "qp_scan_create: instance %d is on another device", i);
+        for (int k = 0; k < i; ++k)
+            QP_REQUIRE(sims[k]->ctx != s->ctx, QP_ERR_ARG,
+                       "qp_scan_create: instances %d and %d share a context (stream)", k, i);
+        bool diag = s->cu3d || s->psi3d;
+        for (qp_field3d *q3 : s->q3s) diag = diag || q3;
+        QP_REQUIRE(!diag, QP_ERR_UNSUPPORTED,
+                   "qp_scan_create: instance %d has 3-D diagnostics (cbq, psi, current); "
+                   "captured slices do not write them", i);
+        QP_REQUIRE(!s->ctx->prof_on, QP_ERR_STATE,
+                   "qp_scan_create: instance %d has kernel timing on", i);
+    }
+    QP_HIP(hipSetDevice(sims[0]->ctx->device));
+    qp_scan *sc = new qp_scan();
+    sc->sims.assign(sims, sims + n);
+    sc->iter_max = iter_max;
+    sc->tol = tol;
+    sc->join.assign(n, nullptr);
+    int rc = QP_OK;
+    hipError_t e = hipStreamCreateWithFlags(&sc->origin, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sc->fork, hipEventDisableTiming);
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&sc->join[i], hipEventDisableTiming);
+    if (e != hipSuccess) rc = hip_fail(e, "qp_scan_create", __FILE__, __LINE__);
+    // the instances' pending work (set-up, begin_step) is not part of the graph
+    for (int i = 0; i < n && !rc; ++i) {
+        e = hipStreamSynchronize(sims[i]->ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+    }
+    sc->key0.assign(n, nullptr);
+    sc->offsets.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        if (!sims[i]->sp.empty()) sc->key0[i] = species_buffer_key(sims[i]->sp[0]);
+        sc->offsets[i] = sims[i]->ctx->tune[QP_TUNE_PUSH] == 1 ? 0 : 1;
+    }
+    // each capture swaps the host pointers once (the push's host side), so
+    // after the two the host is back where it started
+    if (!rc) rc = scan_capture(sc, 0);
+    if (!rc) rc = scan_capture(sc, 1);
+    if (rc) { qp_scan_destroy(sc); return rc; }
+    *out = sc;
+    return QP_OK;
+}
+
+int qp_scan_run(qp_scan *sc, int nslices)
+{
+    QP_REQUIRE(sc && sc->exec[0] && sc->exec[1] && nslices >= 0, QP_ERR_ARG,
+               "qp_scan_run: bad argument");
+    // the buffer parity of every instance (they must agree: an instance
+    // advanced by eager slices in between would not match the graphs)
+    int parity = -1;
+    for (size_t i = 0; i < sc->sims.size(); ++i) {
+        if (!sc->key0[i]) continue;
+        const int p = species_buffer_key(sc->sims[i]->sp[0]) == sc->key0[i] ? 0 : 1;
+        QP_REQUIRE(parity < 0 || p == parity, QP_ERR_STATE,
+                   "qp_scan_run: instance %zu's particles are in the other buffer", i);
+        parity = p;
+    }
+    if (parity < 0) parity = 0;
+    // the slices left in each instance's step (its device slice word)
+    for (size_t i = 0; i < sc->sims.size(); ++i) {
+        qp_sim *s = sc->sims[i];
+        int j = 0;
+        QP_HIP(hipMemcpyAsync(&j, s->ctl.p, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
+        QP_HIP(hipStreamSynchronize(s->ctx->stream));
+        QP_REQUIRE(j >= 1 && j + nslices - 1 <= s->ctx->nz, QP_ERR_ARG,
+                   "qp_scan_run: instance %zu is at slice %d; %d more exceed its %d slices", i, j,
+                   nslices, s->ctx->nz);
+    }
+    // the graph runs after what the instances' streams hold, and they wait
+    // for it in turn, so the library's eager calls before and after order
+    // with the captured slices
+    for (size_t i = 0; i < sc->sims.size(); ++i) {
+        QP_HIP(hipEventRecord(sc->join[i], sc->sims[i]->ctx->stream));
+        QP_HIP(hipStreamWaitEvent(sc->origin, sc->join[i], 0));
+    }
+    for (int k = 0; k < nslices; ++k) {
+        QP_HIP(hipGraphLaunch(sc->exec[parity], sc->origin));
+        scan_flip(sc);
+        parity ^= 1;
+    }
+    QP_HIP(hipEventRecord(sc->fork, sc->origin));
+    for (qp_sim *s : sc->sims) QP_HIP(hipStreamWaitEvent(s->ctx->stream, sc->fork, 0));
+    QP_HIP(hipStreamSynchronize(sc->origin));
+    // the particle move checks of the replayed slices (PPPCHECK2L)
+    for (size_t i = 0; i < sc->sims.size(); ++i)
+        for (qp_species *sp : sc->sims[i]->sp) {
+            int rc = species_check(sp);
+            if (rc) {
+                const std::string msg = qp_last_error();
+                set_error("qp_scan_run: instance %zu: %s", i, msg.c_str());
+                return rc;
+            }
+        }
+    return QP_OK;
+}
+
+int qp_scan_passes(const qp_scan *sc, int instance, int *passes, int nslices)
+{
+    QP_REQUIRE(sc && passes && instance >= 0 && instance < (int)sc->sims.size() && nslices >= 0,
+               QP_ERR_ARG, "qp_scan_passes: bad argument");
+    const qp_sim *s = sc->sims[instance];
+    QP_REQUIRE(nslices <= s->ctx->nz, QP_ERR_ARG, "qp_scan_passes: %d slices > nz", nslices);
+    std::vector<int> h((size_t)s->ctx->nz + 2);
+    QP_HIP(hipMemcpyAsync(h.data(), s->passes_dev.p, h.size() * sizeof(int), hipMemcpyDeviceToHost,
+                          s->ctx->stream));
+    QP_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (int j = 1; j <= nslices; ++j) passes[j - 1] = h[j];
+    return QP_OK;
 }
 
 }  // extern "C"

@@ -7,13 +7,17 @@ U[0.3, 1.5], per-instance beam seed).  The transverse grids are powers of
 two, as the reference's are (indx, indy are log2 sizes,
 simulation_class.f03:151-224), so a scan draws them from 2^indx choices.
 
-run_batched enqueues slice j of every instance before slice j+1 of any, each
-on its instance's stream, so the GPU works on several instances at once; the
-corrector runs the reference's fixed number of passes (no host wait inside a
-slice).  run_sequential runs the instances one after the other.  The two give
-bitwise-equal fields per instance (every kernel is deterministic and an
-instance's work stays on its own stream), which tests/test_gpu_scan.py
-checks.
+run_graph captures one slice of every instance into one HIP graph
+(qpslice.Scan, qp_scan_*: the instances are independent branches, each slice
+index a device word) and replays it once per slice, so the GPU works on all
+instances at once from a single launch per slice; the corrector runs either
+the reference's fixed number of passes or to a residual tolerance (the
+passes after convergence return at once inside the graph).  run_batched
+enqueues slice j of every instance before slice j+1 of any, each on its
+instance's stream, with fixed passes.  run_sequential runs the instances one
+after the other.  All three give bitwise-equal fields per instance (every
+kernel is deterministic and an instance's work stays on its own stream),
+which tests/test_gpu_scan.py checks.
 
 autotune() picks, per instance, the fastest launch shape of each deposit
 kernel (qp_ctx_tune) by timing a few slices of that instance with the
@@ -41,7 +45,7 @@ BOX = (-10.0, 10.0, -10.0, 10.0, 0.0, 10.0)
 
 class Instance:
     def __init__(self, ind, indz, ppc, peak, sr, sz, seed, beam_np, precision=qps.FP32,
-                 iter_=2):
+                 iter_=2, tol=0.0, seed_mode=0):
         self.params = {"grid": 2 ** ind, "ppc": ppc * ppc, "peak": peak, "sigma_r": sr,
                        "sigma_z": sz, "seed": seed}
         self.ctx = qps.Ctx(ind, ind, indz, box=BOX, precision=precision)
@@ -50,14 +54,16 @@ class Instance:
                            sigma_v=(1.0, 1.0, 0.0), np_=beam_np, gamma=2.0e4, peak=peak,
                            dt=10.0, seed=seed)
         self.sim = qps.Sim(self.ctx, [self.sp], [self.bm], iter_=iter_)
-        self.iter = iter_
+        self.iter, self.tol = iter_, tol
+        if tol > 0.0:
+            self.ctx.tune(qps.TUNE_SEED, seed_mode)
 
     def begin(self):
         self.sp.renew(0.0)
         self.sim.begin_step()
 
     def slice(self, j):
-        self.sim.slice(j, iter_max=self.iter, tol=0.0)
+        return self.sim.slice(j, iter_max=self.iter, tol=self.tol)[0]
 
     def fields(self):
         F = self.sim.field
@@ -68,7 +74,8 @@ class Instance:
         self.ctx.close()
 
 
-def draw(n, grids, ppcs, indz, beam_np, seed=0, precision=qps.FP32):
+def draw(n, grids, ppcs, indz, beam_np, seed=0, precision=qps.FP32, iter_=2, tol=0.0,
+         seed_mode=0):
     """n instances with grids 2^g (g from grids), ppc^2 particles per cell
     (ppc from ppcs) and drivers drawn from the ranges above."""
     rng = np.random.default_rng(seed)
@@ -79,7 +86,8 @@ def draw(n, grids, ppcs, indz, beam_np, seed=0, precision=qps.FP32):
         peak, sr, sz = (float(rng.uniform(1, 20)), float(rng.uniform(0.2, 1.0)),
                         float(rng.uniform(0.3, 1.5)))
         out.append(Instance(g, indz, p, peak, sr, sz, seed=1 + 7919 * seed + i,
-                            beam_np=beam_np, precision=precision))
+                            beam_np=beam_np, precision=precision, iter_=iter_, tol=tol,
+                            seed_mode=seed_mode))
     return out
 
 
@@ -122,16 +130,41 @@ def run_batched(insts, nslices):
 
 
 def run_sequential(insts, nslices):
+    """Returns the wall time; each instance's passes per slice go to
+    x.passes."""
     for x in insts:
         x.begin()
     for x in insts:
         x.ctx.sync()
     t = time.perf_counter()
     for x in insts:
-        for j in range(1, nslices + 1):
-            x.slice(j)
+        x.passes = [x.slice(j) for j in range(1, nslices + 1)]
         x.ctx.sync()
     return time.perf_counter() - t
+
+
+def run_graph(insts, nslices, chunk=None):
+    """All instances from one captured slice graph, replayed nslices times
+    (in runs of `chunk` slices); returns the wall time of the replays.  Each
+    instance's passes per slice go to x.passes."""
+    for x in insts:
+        x.begin()
+    scan = qps.Scan([x.sim for x in insts], iter_max=insts[0].iter, tol=insts[0].tol)
+    try:
+        for x in insts:
+            x.ctx.sync()
+        t = time.perf_counter()
+        done = 0
+        while done < nslices:
+            k = min(chunk or nslices, nslices - done)
+            scan.run(k)
+            done += k
+        el = time.perf_counter() - t
+        for i, x in enumerate(insts):
+            x.passes = [int(p) for p in scan.passes(i, nslices)]
+    finally:
+        scan.close()
+    return el
 
 
 def main(argv=None):
@@ -144,25 +177,49 @@ def main(argv=None):
     ap.add_argument("--indz", type=int, default=8)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--beam-np", type=int, nargs=3, default=[32, 32, 256])
+    ap.add_argument("--tol", type=float, default=1e-6,
+                    help="corrector residual tolerance (0: the reference's fixed passes)")
+    ap.add_argument("--iter-max", type=int, default=20)
+    ap.add_argument("--seed-mode", type=int, default=1, choices=(0, 1),
+                    help="corrector seed with a tolerance: 0 previous slice, 1 extrapolated")
+    ap.add_argument("--fp64", action="store_true", help="FP64 instances (default FP32)")
     ap.add_argument("--tune", action="store_true", help="autotune each instance first")
+    ap.add_argument("--no-batched", dest="batched", action="store_false",
+                    help="skip the round-robin stream run (fixed passes only)")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.time()
-    insts = draw(args.n, args.grids, args.ppcs, args.indz, tuple(args.beam_np), args.seed)
+    iter_ = args.iter_max if args.tol > 0 else 2
+    insts = draw(args.n, args.grids, args.ppcs, args.indz, tuple(args.beam_np), args.seed,
+                 precision=qps.FP64 if args.fp64 else qps.FP32, iter_=iter_, tol=args.tol,
+                 seed_mode=args.seed_mode)
     log(f"[scan] {args.n} instances set up in {time.time() - t0:.1f}s")
     if args.tune:
         for x in insts:
             x.begin()
             x.params["tune"] = autotune(x, 1)
-    ts = run_sequential(insts, args.slices)
-    tb = run_batched(insts, args.slices)
     total = args.n * args.slices
-    res = {"instances": args.n, "slices_each": args.slices, "sequential_slices_per_s": total / ts,
-           "batched_slices_per_s": total / tb, "speedup": ts / tb,
-           "grids": sorted({x.params["grid"] for x in insts})}
-    log(f"[scan] sequential {total / ts:.1f} slices/s, batched {total / tb:.1f} slices/s "
-        f"({ts / tb:.2f}x)")
+    ts = run_sequential(insts, args.slices)
+    seq_passes = [list(x.passes) for x in insts]
+    log(f"[scan] sequential {total / ts:.1f} slices/s")
+    tg = run_graph(insts, args.slices)
+    same = all(list(x.passes) == p for x, p in zip(insts, seq_passes))
+    log(f"[scan] graph {total / tg:.1f} slices/s ({ts / tg:.2f}x), passes as sequential: {same}")
+    res = {"instances": args.n, "slices_each": args.slices,
+           "precision": "fp64" if args.fp64 else "fp32",
+           "corrector": (f"tolerance {args.tol:g} (max {iter_} passes), "
+                         + ("extrapolated" if args.seed_mode else "previous-slice") + " seed")
+           if args.tol > 0 else "2 fixed passes",
+           "sequential_slices_per_s": total / ts, "graph_slices_per_s": total / tg,
+           "graph_over_sequential": ts / tg, "graph_passes_equal_sequential": same,
+           "passes_mean": float(np.mean(seq_passes)),
+           "grids": sorted({x.params["grid"] for x in insts}),
+           "ppc": sorted({x.params["ppc"] for x in insts})}
+    if args.batched and args.tol <= 0:
+        tb = run_batched(insts, args.slices)
+        res["batched_streams_slices_per_s"] = total / tb
+        log(f"[scan] round-robin streams {total / tb:.1f} slices/s")
     for x in insts:
         x.close()
     import json

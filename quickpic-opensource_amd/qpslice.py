@@ -604,6 +604,40 @@ class Sim:
             pass
 
 
+class Scan:
+    """One HIP graph holding one xi slice of each simulation (qp_scan_*):
+    run(n) replays it n times, every simulation continuing from its own
+    last slice.  Create it after each simulation's begin_step."""
+
+    def __init__(self, sims, iter_max=2, tol=0.0):
+        self.sims = list(sims)
+        hs = (_P * len(self.sims))(*[x.h for x in self.sims])
+        self.h = _P()
+        _call(lib().qp_scan_create, hs, len(self.sims), iter_max, _D(tol), C.byref(self.h))
+        for x in self.sims:   # closing any of the contexts closes the scan first
+            x.ctx._adopt(self)
+
+    def run(self, nslices):
+        _call(lib().qp_scan_run, self.h, nslices)
+
+    def passes(self, i, nslices):
+        """Corrector passes of slices 1..nslices of simulation i."""
+        a = np.zeros(nslices, dtype=np.int32)
+        _call(lib().qp_scan_passes, self.h, i, a.ctypes.data_as(C.c_void_p), nslices)
+        return a
+
+    def close(self):
+        if self.h:
+            lib().qp_scan_destroy(self.h)
+        self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def exported_symbols(header=os.path.join(os.path.dirname(_HERE), "include", "qpslice.h")):
     """Names of the functions include/qpslice.h declares."""
     import re

@@ -171,6 +171,126 @@ def test_unet_forward_backward_against_oracle():
         assert np.isfinite(g).all() and np.abs(g).max() > 0
 
 
+def test_unet_backward_every_layer_against_oracle():
+    """Every layer's backward from the GPU's own activations and incoming
+    gradient: the weight and bias gradients of all 14 convolutions and the
+    data gradients through each convolution, pooling and upsampling, with
+    the skip connections' sums (the gradient of an encoder output is its
+    skip part plus the pooled path's)."""
+    banner("unet backward, every layer")
+    H = W = 32
+    rng = np.random.default_rng(21)
+    net = qpnet.UNet(seed=5)
+    for cv in net.enc + net.dec:
+        cv.b.copy_(torch.from_numpy(rng.standard_normal(cv.cout).astype(np.float32) * 0.05))
+    planes = torch.from_numpy(rng.standard_normal((qpnet.NIN, H, W)).astype(np.float32)).cuda()
+    out = net.forward(planes.flatten(), H * W, W, H, W).cpu().numpy().astype(np.float64)
+    tgt = rng.standard_normal(out.shape).astype(np.float32)
+    net.backward(torch.from_numpy(tgt).cuda())
+    torch.cuda.synchronize()
+    c1, c2, c3, c4 = qpnet.CH
+    dims = {"x0": (H, W, 16), "e1a": (H, W, c1), "cat1": (H, W, c2 + c1),
+            "p1": (H // 2, W // 2, c1), "e2a": (H // 2, W // 2, c2),
+            "cat2": (H // 2, W // 2, c3 + c2), "p2": (H // 4, W // 4, c2),
+            "e3a": (H // 4, W // 4, c3), "cat3": (H // 4, W // 4, c4 + c3),
+            "p3": (H // 8, W // 8, c3), "ba": (H // 8, W // 8, c4), "bb": (H // 8, W // 8, c4),
+            "d3a": (H // 4, W // 4, c3), "d3b": (H // 4, W // 4, c3),
+            "d2a": (H // 2, W // 2, c2), "d2b": (H // 2, W // 2, c2),
+            "d1a": (H, W, c1), "d1b": (H, W, c1)}
+    A = {k: host(net.a[k], v) for k, v in dims.items()}
+    G = {k: host(net.g[k], v) for k, v in dims.items()}
+    e, d = net.enc, net.dec
+    worst = {}
+
+    def layer(name, cv, x, y, gy):
+        """numpy gradients of conv cv from the GPU's x, y and gy; checks
+        the GPU's weight and bias gradients; returns the data gradient."""
+        gx, gw, gb = nr.conv3x3_bwd(x, bf(cv.w.cpu().numpy()), y, gy)
+        ew = rel(cv.gw.cpu().numpy(), gw)
+        eb = rel(cv.gb.cpu().numpy(), gb)
+        worst[name] = max(ew, eb)
+        assert ew < 2e-2 and eb < 2e-2, (name, ew, eb)
+        return gx
+
+    def same(name, got, want, bar=3e-2):
+        err = rel(got, want)
+        worst[name] = max(worst.get(name, 0.0), err)
+        assert err < bar, (name, err)
+
+    gx = layer("dec5", d[5], A["d1a"], A["d1b"], G["d1b"])
+    same("g_d1a", G["d1a"], gx)
+    gx = layer("dec4", d[4], A["cat1"], A["d1a"], G["d1a"])
+    skip1 = gx[:, :, c2:]
+    same("g_d2b", G["d2b"], nr.up2_bwd(G["cat1"][:, :, :c2]))
+    same("g_cat1_up", G["cat1"][:, :, :c2], gx[:, :, :c2])
+    gx = layer("dec3", d[3], A["d2a"], A["d2b"], G["d2b"])
+    same("g_d2a", G["d2a"], gx)
+    gx = layer("dec2", d[2], A["cat2"], A["d2a"], G["d2a"])
+    skip2 = gx[:, :, c3:]
+    same("g_cat2_up", G["cat2"][:, :, :c3], gx[:, :, :c3])
+    same("g_d3b", G["d3b"], nr.up2_bwd(G["cat2"][:, :, :c3]))
+    gx = layer("dec1", d[1], A["d3a"], A["d3b"], G["d3b"])
+    same("g_d3a", G["d3a"], gx)
+    gx = layer("dec0", d[0], A["cat3"], A["d3a"], G["d3a"])
+    skip3 = gx[:, :, c4:]
+    same("g_cat3_up", G["cat3"][:, :, :c4], gx[:, :, :c4])
+    same("g_bb", G["bb"], nr.up2_bwd(G["cat3"][:, :, :c4]))
+    gx = layer("enc7", e[7], A["ba"], A["bb"], G["bb"])
+    same("g_ba", G["ba"], gx)
+    gx = layer("enc6", e[6], A["p3"], A["ba"], G["ba"])
+    same("g_p3", G["p3"], gx)
+    # encoder level 3 output: skip gradient plus the pooled path's
+    same("g_e3", G["cat3"][:, :, c4:], skip3 + nr.pool2_bwd(G["p3"]))
+    gx = layer("enc5", e[5], A["e3a"], A["cat3"][:, :, c4:], G["cat3"][:, :, c4:])
+    same("g_e3a", G["e3a"], gx)
+    gx = layer("enc4", e[4], A["p2"], A["e3a"], G["e3a"])
+    same("g_p2", G["p2"], gx)
+    same("g_e2", G["cat2"][:, :, c3:], skip2 + nr.pool2_bwd(G["p2"]))
+    gx = layer("enc3", e[3], A["e2a"], A["cat2"][:, :, c3:], G["cat2"][:, :, c3:])
+    same("g_e2a", G["e2a"], gx)
+    gx = layer("enc2", e[2], A["p1"], A["e2a"], G["e2a"])
+    same("g_p1", G["p1"], gx)
+    same("g_e1", G["cat1"][:, :, c2:], skip1 + nr.pool2_bwd(G["p1"]))
+    gx = layer("enc1", e[1], A["e1a"], A["cat1"][:, :, c2:], G["cat1"][:, :, c2:])
+    same("g_e1a", G["e1a"], gx)
+    layer("enc0", e[0], A["x0"], A["e1a"], G["e1a"])
+    print("worst relative error per layer:",
+          " ".join(f"{k}={v:.1e}" for k, v in worst.items()), flush=True)
+
+
+def test_network_fits_a_fixed_set_of_input_dependent_targets():
+    """Overfitting check of the whole training path (forward, backward,
+    Adam): 4 fixed random inputs whose targets are a local function of
+    each input (a 3x3 stencil of two channels and a product of two others,
+    so the output depends on the input, not only on the biases).  Adam on
+    the 4 samples in turn must bring the mean loss below 10 % of where it
+    started."""
+    banner("unet overfit")
+    H = W = 32
+    rng = np.random.default_rng(31)
+    net = qpnet.UNet(seed=6)
+    xs, ys = [], []
+    for _ in range(4):
+        x = rng.standard_normal((qpnet.NIN, H, W))
+        # smooth inputs (the seed's inputs are smooth fields)
+        for _ in range(2):
+            x = 0.25 * (np.roll(x, 1, 1) + np.roll(x, -1, 1) + np.roll(x, 1, 2) + np.roll(x, -1, 2))
+        x /= x.std(axis=(1, 2), keepdims=True)
+        t0 = 0.5 * x[0] + 0.25 * (np.roll(x[2], 1, 0) - np.roll(x[2], -1, 0))
+        t1 = 0.5 * x[1] * x[4] - 0.3 * x[7]
+        xs.append(torch.from_numpy(x.astype(np.float32)).cuda().flatten())
+        ys.append(torch.from_numpy(np.stack([t0, t1]).reshape(2, -1).astype(np.float32)).cuda())
+    losses = []
+    for it in range(600):
+        k = it % 4
+        net.forward(xs[k], H * W, W, H, W)
+        losses.append(net.backward(ys[k]))
+        net.adam(lr=1e-3)
+    first, last = float(np.mean(losses[:4])), float(np.mean(losses[-4:]))
+    print(f"overfit: mean loss {first:.4g} -> {last:.4g}", flush=True)
+    assert last < 0.1 * first
+
+
 def test_training_reduces_the_loss():
     banner("unet training")
     H = W = 32
