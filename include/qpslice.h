@@ -234,6 +234,16 @@ int qp_species_renew(qp_species *sp, double s);                   /* :138 */
 int qp_species_set_longitudinal(qp_species *sp, int n, const double *s_pts,
                                 const double *fs, double s_now);
 int qp_species_qdp(qp_species *sp, qp_field2d *q);                /* :166 */
+/* qp_species_qdp without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The deposit's tile-border halo lives in a
+ * static device array of the library (2 MiB per GPU, held until the process
+ * ends) instead of the context's buffer, so a grid's halo (64 border nodes
+ * per 16 x 16-cell tile) must fit it: up to 4096 tiles in FP64 (1024 x 1024)
+ * and 8192 in FP32 (2048 x 1024); larger grids return QP_ERR_UNSUPPORTED.
+ * Results equal qp_species_qdp's.  Calls that overlap on one GPU (other
+ * threads, or unordered streams) share the array and race. */
+int qp_species_qdp_noalloc(qp_species *sp, qp_field2d *q);
 int qp_species_extpsi(qp_species *sp, const qp_field2d *psi, double dex);
 int qp_species_amjdp(qp_species *sp, const qp_field2d *ef,
                      const qp_field2d *bf, const qp_field2d *psit,

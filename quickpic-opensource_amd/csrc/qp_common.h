@@ -35,6 +35,19 @@ constexpr int FLAG_WORDS = 64;          // converged flags (passes 0..63)
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
+// Scratch of qp_species_qdp_noalloc: the charge deposit's tile-border halo
+// (one component) in a static device array, for callers that cannot
+// allocate: 2 MiB per GPU, held until the process ends, which holds the
+// halo of NOALLOC_TILES tiles of 16 x 16 cells in FP64 (a 1024 x 1024 grid)
+// and of twice as many in FP32.
+constexpr size_t NOALLOC_TILES = 4096;
+constexpr size_t NOALLOC_BYTES = NOALLOC_TILES * HALO * sizeof(double);
+constexpr int NOALLOC_DEVICES = 64;
+// Looks the array's device address up once for the current device (device);
+// qp_ctx_create calls it with its device current.
+int noalloc_setup(int device);
+// The address noalloc_setup found for device, or null.
+void *noalloc_scratch(int device);
 
 #define QP_HIP(call)                                                        \
     do {                                                                    \

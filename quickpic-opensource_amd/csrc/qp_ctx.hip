@@ -3,9 +3,40 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <mutex>
+
 #include "qp_common.h"
 
+// The scratch array of qp_species_qdp_noalloc.  Host code never uses this
+// name as an address: each GPU holds its own copy, whose device address
+// noalloc_setup looks up with hipGetSymbolAddress and keeps per device.
+static __device__ __align__(256) unsigned char scratch[qp::NOALLOC_BYTES];
+
 namespace qp {
+
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[NOALLOC_DEVICES];
+
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_DEVICES) return QP_OK;   // the entry point refuses it
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return QP_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return hip_fail(e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)",
+                        __FILE__, __LINE__);
+    g_noalloc_ptr[device] = ptr;
+    return QP_OK;
+}
+
+void *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= NOALLOC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
 
 static thread_local char g_err[512] = "";
 
@@ -210,6 +241,10 @@ int qp_ctx_create(int indx, int indy, int indz, const double box[6], int precisi
     QP_REQUIRE(precision == QP_FP64 || precision == QP_FP32, QP_ERR_ARG,
                "qp_ctx_create: bad precision %d", precision);
     QP_HIP(hipSetDevice(device));
+    {
+        int rc0 = qp::noalloc_setup(device);   // with the context's device current
+        if (rc0) return rc0;
+    }
     qp_ctx *c = new qp_ctx();
     c->indx = indx; c->indy = indy; c->indz = indz;
     c->nx = 1 << indx; c->ny = 1 << indy; c->nz = 1 << indz;

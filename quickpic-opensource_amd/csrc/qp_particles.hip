@@ -1836,6 +1836,37 @@ int qp_species_qdp(qp_species *sp, qp_field2d *q)
     return rc;
 }
 
+// qp_species_qdp with the deposit's tile-border halo in the library's static
+// scratch array instead of the context's buffer: no allocation of any kind.
+// Every call on a GPU shares that GPU's array, so calls on one GPU that
+// overlap (other threads, or other streams without ordering) race.  The
+// halo is fully rewritten by every deposit before it is read, so nothing
+// has to be cleared between calls.
+int qp_species_qdp_noalloc(qp_species *sp, qp_field2d *q)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(q && q->dim == 1 && q->state == 0, QP_ERR_STATE,
+               "qdp_noalloc: q must be real space, dim 1");
+    qp_ctx *c = sp->ctx;
+    const size_t need = (size_t)c->ntiles * HALO * c->esize;
+    QP_REQUIRE(need <= qp::NOALLOC_BYTES, QP_ERR_UNSUPPORTED,
+               "qdp_noalloc: %d tiles need %zu B of scratch; the maximum is %zu B "
+               "(%zu tiles in this precision)",
+               c->ntiles, need, qp::NOALLOC_BYTES, qp::NOALLOC_BYTES / (HALO * c->esize));
+    void *halo = qp::noalloc_scratch(c->device);
+    QP_REQUIRE(halo, QP_ERR_UNSUPPORTED, "qdp_noalloc: no scratch address for device %d",
+               c->device);
+    int rc = c->prec == QP_FP64
+                 ? deposit_q_t<double>(sp, sp->q->rs.as<double>(), static_cast<double *>(halo))
+                 : deposit_q_t<float>(sp, sp->q->rs.as<float>(), static_cast<float *>(halo));
+    if (rc) return rc;
+    sp->q->state = 0;
+    sp->q->gcells = 1;
+    rc = qp_field2d_add(q, sp->q, q, 0, nullptr, nullptr, nullptr);
+    if (!rc) rc = qp_field2d_add(q, sp->qn, q, 0, nullptr, nullptr, nullptr);
+    return rc;
+}
+
 int qp_species_extpsi(qp_species *sp, const qp_field2d *psi, double dex)
 {
     SPCHECK(sp);

@@ -366,6 +366,11 @@ class Species:
     def qdp(self, q):
         _call(lib().qp_species_qdp, self.h, q.h)
 
+    def qdp_noalloc(self, q):
+        """qdp with the deposit's scratch in the library's static array (no
+        allocation; grids up to 4096 tiles in FP64, 8192 in FP32)."""
+        _call(lib().qp_species_qdp_noalloc, self.h, q.h)
+
     def extpsi(self, psi, dex):
         _call(lib().qp_species_extpsi, self.h, psi.h, _D(dex))
 
