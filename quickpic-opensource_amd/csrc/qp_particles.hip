@@ -63,14 +63,26 @@ struct TileGeom {
     int nx, ny, mx1, my1, ldr;
 };
 
+// Particle positions.  FP64 records hold x and y in grid units, as the
+// reference does.  FP32 records hold them relative to the origin of the
+// particle's 16 x 16-cell tile (the tile is implied by the sorted order):
+// a float near 1000 resolves 6e-5 of a cell, one in [0, 16) 1e-6, and the
+// plasma's response ahead of a driver is a displacement of that order per
+// slice.  Downloads and uploads convert at the boundary (in FP64), and the
+// push re-bases a particle that changes tile.
+template <typename R> struct TileRel {
+    static constexpr bool on = std::is_same<R, float>::value;
+};
+
 // v mod m for -m <= v < 2m (a tile index plus a direction in {-1, 0, 1}):
 // the periodic neighbour without an integer division
 __device__ __forceinline__ int wrap1(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
 
-__device__ __forceinline__ int cell_key(int cx, int cy, int mx1)
+__host__ __device__ __forceinline__ int cell_key(int cx, int cy, int mx1)
 {
     return ((cy >> 4) * mx1 + (cx >> 4)) * TCELLS + ((cy & 15) << 4) + (cx & 15);
 }
+static inline int host_cell_key(int cx, int cy, int mx1) { return cell_key(cx, cy, mx1); }
 
 // this thread's cell: key = blockIdx.x*256 + threadIdx.x
 __device__ __forceinline__ void my_cell(const TileGeom &g, int &cx, int &cy)
@@ -280,7 +292,8 @@ k_deposit_q(TileGeom g, Soa<R> s, const int *__restrict__ off, DepOut<R> out, R 
         const int key = kb + lrow * TILE + lcol;
         const int i0 = off[key], i1 = off[key + 1];
         const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
-        const R fx = (R)(tx * TILE + lcol), fy = (R)(ty * TILE + lrow);
+        const R fx = (R)(TileRel<R>::on ? lcol : tx * TILE + lcol);
+        const R fy = (R)(TileRel<R>::on ? lrow : ty * TILE + lrow);
         R a00 = 0, a10 = 0, a01 = 0, a11 = 0;
         for (int c0 = G0; c0 < G1; c0 += CH) {
             const int n = min(CH, G1 - c0);
@@ -358,15 +371,35 @@ __device__ __forceinline__ void p67_from_psi(R dx, R dx2, R vx, R vy, R &p6, R &
 
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R dex)
+k_extpsi(int64_t np, int nx, int ny, int ldr, Soa<R> s, const R *psi, R qbm, R dex,
+         const int *__restrict__ off, int mx1, int ntiles)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= np) return;
     const R x = s.x[i], y = s.y[i];
-    const int cx = min(max((int)x, 0), nx - 1), cy = min(max((int)y, 0), ny - 1);
+    int cx, cy;
+    R dxp, dyp;
+    if constexpr (TileRel<R>::on) {
+        // the particle's tile: the last one starting at or before i
+        int lo = 0, hi = ntiles - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid * TCELLS] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const int lx = min(max((int)x, 0), TILE - 1), ly = min(max((int)y, 0), TILE - 1);
+        cx = (lo % mx1) * TILE + lx;
+        cy = (lo / mx1) * TILE + ly;
+        dxp = x - (R)lx;
+        dyp = y - (R)ly;
+    } else {
+        cx = min(max((int)x, 0), nx - 1);
+        cy = min(max((int)y, 0), ny - 1);
+        dxp = x - (R)cx;
+        dyp = y - (R)cy;
+    }
     const Corners<R> f = load_corners(psi, ldr, cx, cy);
     const R dx2 = dex * dex;
-    R dxp = x - (R)cx, dyp = y - (R)cy;
     R amx = R(1) - dxp, amy = R(1) - dyp;
     R vx = s.px[i], vy = s.py[i];
     R dx = gather(f, amx, amy, dxp, dyp);
@@ -533,7 +566,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
         Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
         prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
-        const R fx = (R)cx, fy = (R)cy;
+        const R fx = (R)(TileRel<R>::on ? lcol : cx), fy = (R)(TileRel<R>::on ? lrow : cy);
         R acc[8][4];
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -645,7 +678,7 @@ k_amjdp_pf(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, cons
         Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
         prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
-        const R fx = (R)cx, fy = (R)cy;
+        const R fx = (R)(TileRel<R>::on ? lcol : cx), fy = (R)(TileRel<R>::on ? lrow : cy);
         R acc[8][4];
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -769,7 +802,7 @@ k_amjdp_dma(TileGeom g, Soa<float> s, const int *__restrict__ off, const float *
         Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
         prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
-        const R fx = (R)cx, fy = (R)cy;
+        const R fx = (R)(TileRel<R>::on ? lcol : cx), fy = (R)(TileRel<R>::on ? lrow : cy);
         R acc[8][4];
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -876,7 +909,7 @@ k_amjdp_w(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const
         Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
         prep_fields(F);
         const Corners<R> FP = psi ? load_corners(psi, g.ldr, cx, cy) : Corners<R>{};
-        const R fx = (R)cx, fy = (R)cy;
+        const R fx = (R)(TileRel<R>::on ? lcol : cx), fy = (R)(TileRel<R>::on ? lrow : cy);
         R acc[8][4];
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -1033,11 +1066,15 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
         int b = 0, dcell = 0;
         R ox_ = R(0), oy_ = R(0), opx = R(0), opy = R(0), opz = R(0);
         if (valid) {
-            const int cx = min(max((int)x, 0), g.nx - 1), cy = min(max((int)y, 0), g.ny - 1);
+            // absolute cell (FP64) or cell within the tile (FP32 tile-relative)
+            const int cx = TileRel<R>::on ? min(max((int)x, 0), TILE - 1)
+                                          : min(max((int)x, 0), g.nx - 1);
+            const int cy = TileRel<R>::on ? min(max((int)y, 0), TILE - 1)
+                                          : min(max((int)y, 0), g.ny - 1);
             // local node of the cell, clamped into the tile (a particle flagged
             // by the move check may sit outside it; its result is reported)
-            const int li = min(max(cx - tx0 * TILE, 0), TILE - 1);
-            const int lj = min(max(cy - ty0 * TILE, 0), TILE - 1);
+            const int li = TileRel<R>::on ? cx : min(max(cx - tx0 * TILE, 0), TILE - 1);
+            const int lj = TileRel<R>::on ? cy : min(max(cy - ty0 * TILE, 0), TILE - 1);
             const int m = lj * 17 + li;
             auto lc = [&](int f) {
                 return Corners<R>{fnode[f][m], fnode[f][m + 1], fnode[f][m + 17], fnode[f][m + 18]};
@@ -1088,39 +1125,65 @@ k_push(TileGeom g, Soa<R> s, Soa<R> o, const int *__restrict__ off,
             R dtc1 = divq(dt, sqrtq(R(1) + (dx * dx + dy * dy + dz * dz) * dex * dex) - dz * dex);
             dx = x + dx * dtc1;
             dy = y + dy * dtc1;
-            int dirx = 0, diry = 0;
-            if (dx >= edgerx) dirx = 1;
-            else if (dx < edgelx) {
-                dirx = -1;
-                if (dx < R(0) && dx + anx >= anx) { dx = R(0); dirx = 0; }
+            int dirx = 0, diry = 0, ncx, ncy;
+            bool far = false;   // beyond the neighbouring tiles (PPPCHECK2L)
+            if constexpr (TileRel<R>::on) {
+                // A position that leaves [0, 16) moves to the neighbouring
+                // tile and is re-based by 16 (exact for -16 <= x < 32).  One
+                // just below 0 whose re-based value rounds to 16 stays at 0,
+                // as the absolute form keeps a particle whose wrap rounds to
+                // the box edge.  Tile indices wrap periodically below.
+                const R T = (R)TILE;
+                if (dx >= T) { dirx = 1; dx = dx - T; }
+                else if (dx < R(0)) {
+                    if (dx + T >= T) dx = R(0);
+                    else { dirx = -1; dx = dx + T; }
+                }
+                if (dy >= T) { diry = 1; dy = dy - T; }
+                else if (dy < R(0)) {
+                    if (dy + T >= T) dy = R(0);
+                    else { diry = -1; dy = dy + T; }
+                }
+                far = !(dx >= R(0) && dx < T && dy >= R(0) && dy < T);
+                if (far && !(isfinite(dx) && isfinite(dy)) && MODE != 2) atomicMax(err, 1 << 30);
+                ncx = far ? min(max((int)dx, 0), TILE - 1) : (int)dx;
+                ncy = far ? min(max((int)dy, 0), TILE - 1) : (int)dy;
+            } else {
+                if (dx >= edgerx) dirx = 1;
+                else if (dx < edgelx) {
+                    dirx = -1;
+                    if (dx < R(0) && dx + anx >= anx) { dx = R(0); dirx = 0; }
+                }
+                if (dy >= edgery) diry = 1;
+                else if (dy < edgely) {
+                    diry = -1;
+                    if (dy < R(0) && dy + any >= any) { dy = R(0); diry = 0; }
+                }
+                if (dx < R(0)) dx = dx + anx;
+                if (dx >= anx) dx = dx - anx;
+                if (dy < R(0)) dy = dy + any;
+                if (dy >= any) dy = dy - any;
+                ncx = (int)dx;
+                ncy = (int)dy;
+                // a non-finite or out-of-range position must not become an index
+                if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
+                    if (MODE != 2) atomicMax(err, 1 << 30);
+                    ncx = min(max(ncx, 0), g.nx - 1);
+                    ncy = min(max(ncy, 0), g.ny - 1);
+                }
             }
-            if (dy >= edgery) diry = 1;
-            else if (dy < edgely) {
-                diry = -1;
-                if (dy < R(0) && dy + any >= any) { dy = R(0); diry = 0; }
-            }
-            if (dx < R(0)) dx = dx + anx;
-            if (dx >= anx) dx = dx - anx;
-            if (dy < R(0)) dy = dy + any;
-            if (dy >= any) dy = dy - any;
             if (MODE == 0) {
                 s.x[i] = dx;
                 s.y[i] = dy;
             }
             ox_ = dx;
             oy_ = dy;
-            int ncx = (int)dx, ncy = (int)dy;
-            // a non-finite or out-of-range position must not become an index
-            if (!(ncx >= 0 && ncx < g.nx && ncy >= 0 && ncy < g.ny)) {
-                if (MODE != 2) atomicMax(err, 1 << 30);
-                ncx = min(max(ncx, 0), g.nx - 1);
-                ncy = min(max(ncy, 0), g.ny - 1);
-            }
             const int txd = wrap1(tx0 + dirx, g.mx1), tyd = wrap1(ty0 + diry, g.my1);
             // destination tile: the neighbour in the direction of motion, or,
             // for a particle the move check flags, its own tile
             int dt = tyd * g.mx1 + txd;
-            if ((ncx >> 4) != txd || (ncy >> 4) != tyd) {
+            if (!TileRel<R>::on) far = (ncx >> 4) != txd || (ncy >> 4) != tyd;
+            if (far) {
                 if (MODE != 2) atomicMax(err, 1 + txd + g.mx1 * tyd);
                 dirx = diry = 0;
                 dt = S;
@@ -1258,6 +1321,15 @@ __global__ void k_keys(int64_t np, const R *x, const R *y, int nx, int ny, int m
     atomicAdd(&count[k], 1);
 }
 
+// the same for keys computed on the host (FP32 tile-relative records)
+__global__ void k_count_keys(int64_t np, const int *key, int *idx, int *count)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    idx[i] = (int)i;
+    atomicAdd(&count[key[i]], 1);
+}
+
 // b[p] = a[src[p]]
 template <typename R>
 __global__ void k_gather(int64_t np, Soa<R> a, Soa<R> b, const int *src)
@@ -1338,9 +1410,11 @@ static int check_err(qp_species *sp, const char *what)
 }
 
 // sort an arbitrary particle order by cell: keys, a stable radix sort of
-// (key, index), offsets, then a gather into the alternate arrays
+// (key, index), offsets, then a gather into the alternate arrays.  hkeys:
+// the cell keys, computed on the host (FP32 records hold tile-relative
+// positions, from which the device cannot tell the tile)
 template <typename R>
-static int rebin_unsorted(qp_species *sp)
+static int rebin_unsorted(qp_species *sp, const std::vector<int> *hkeys = nullptr)
 {
     qp_ctx *c = sp->ctx;
     int rc = zero_counts(sp);
@@ -1358,11 +1432,18 @@ static int rebin_unsorted(qp_species *sp)
                                               bits, c->stream));
     rc = tmp.alloc(tb + 256);
     if (rc) return rc;
-    {
-    ProfScope _ps(c, QP_PROF_SORT, 2.0 * sizeof(R) * (double)np);
-    hipLaunchKernelGGL(k_keys<R>, pgrid(np), dim3(256), 0, c->stream, np, sp->a[0].as<R>(),
-                       sp->a[1].as<R>(), c->nx, c->ny, c->mx1, sp->key.as<int>(),
-                       sp->slot.as<int>(), sp->count.as<int>(), sp->err.as<int>());
+    if (hkeys) {
+        QP_REQUIRE((int64_t)hkeys->size() == np, QP_ERR_ARG, "particle load: %zu keys for %lld",
+                   hkeys->size(), (long long)np);
+        QP_HIP(hipMemcpyAsync(sp->key.p, hkeys->data(), (size_t)np * sizeof(int),
+                              hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_count_keys, pgrid(np), dim3(256), 0, c->stream, np,
+                           sp->key.as<const int>(), sp->slot.as<int>(), sp->count.as<int>());
+    } else {
+        ProfScope _ps(c, QP_PROF_SORT, 2.0 * sizeof(R) * (double)np);
+        hipLaunchKernelGGL(k_keys<R>, pgrid(np), dim3(256), 0, c->stream, np, sp->a[0].as<R>(),
+                           sp->a[1].as<R>(), c->nx, c->ny, c->mx1, sp->key.as<int>(),
+                           sp->slot.as<int>(), sp->count.as<int>(), sp->err.as<int>());
     }
     QP_LAUNCH_CHECK();
     QP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, sp->key.as<int>(), keys2.as<int>(),
@@ -1502,13 +1583,23 @@ static int distribute(qp_species *sp)
     const double qm = sp->den_factor * sp->p.density * sp->p.q / std::fabs(sp->p.q) / (double)xppc /
                       (double)yppc;
     std::vector<R> hx((size_t)np), hy((size_t)np);
+    std::vector<int> hkeys;
+    if (TileRel<R>::on) hkeys.resize((size_t)np);
     int64_t n = 0;
     for (int i = 2; i <= nx - 1; ++i)
         for (int j = 2; j <= ny; ++j)
             for (int ix = 0; ix < xppc; ++ix)
                 for (int iy = 0; iy < yppc; ++iy) {
-                    hx[n] = (R)((ix + 0.5) / xppc + i - 1);
-                    hy[n] = (R)((iy + 0.5) / yppc + j - 1);
+                    const double x = (ix + 0.5) / xppc + i - 1, y = (iy + 0.5) / yppc + j - 1;
+                    if (TileRel<R>::on) {
+                        const int cx = (int)x, cy = (int)y;
+                        hkeys[n] = host_cell_key(cx, cy, c->mx1);
+                        hx[n] = (R)(x - (double)(cx & ~(TILE - 1)));
+                        hy[n] = (R)(y - (double)(cy & ~(TILE - 1)));
+                    } else {
+                        hx[n] = (R)x;
+                        hy[n] = (R)y;
+                    }
                     ++n;
                 }
     sp->np = np;
@@ -1523,7 +1614,7 @@ static int distribute(qp_species *sp)
         int rc = launch_fill(c, sp->a[k].p, (size_t)np, fillv[k - 2]);
         if (rc) return rc;
     }
-    int rc = rebin_unsorted<R>(sp);
+    int rc = rebin_unsorted<R>(sp, TileRel<R>::on ? &hkeys : nullptr);
     if (rc) return rc;
     return set_q_uniform(sp, true, qm);
 }
@@ -1599,7 +1690,8 @@ static int extpsi_t(qp_species *sp, const qp_field2d *psi, double dex)
     {
     ProfScope _ps(c, QP_PROF_EXTPSI, (6.0 * sp->np + (c->nx + 1.0) * (c->ny + 1.0)) * sizeof(R));
     hipLaunchKernelGGL(k_extpsi<R>, pgrid(sp->np), dim3(256), 0, c->stream, sp->np, c->nx, c->ny,
-                       c->ldr, soa<R>(sp->a), psi->rs.as<R>(), (R)sp->qbm, (R)dex);
+                       c->ldr, soa<R>(sp->a), psi->rs.as<R>(), (R)sp->qbm, (R)dex,
+                       sp->off.as<int>(), c->mx1, c->ntiles);
     }
     QP_LAUNCH_CHECK();
     return QP_OK;
@@ -1946,18 +2038,40 @@ int qp_species_count(const qp_species *sp, int64_t *np)
 
 }  // extern "C"
 
+static int host_cell_counts(const qp_species *sp, std::vector<int> &off);
+
 template <typename R>
 static int download_t(const qp_species *sp, double *part, int64_t ld)
 {
     qp_ctx *c = sp->ctx;
     std::vector<R> h((size_t)sp->np);
+    // FP32 positions are relative to their tile's origin: the tile of each
+    // particle comes from the cell offsets (the records are sorted by tile)
+    std::vector<double> ox, oy;
+    if (TileRel<R>::on) {
+        std::vector<int> off;
+        int rc = host_cell_counts(sp, off);
+        if (rc) return rc;
+        ox.resize((size_t)sp->np);
+        oy.resize((size_t)sp->np);
+        for (int t = 0; t < c->ntiles; ++t)
+            for (int i = off[(size_t)t * TCELLS]; i < off[(size_t)(t + 1) * TCELLS]; ++i) {
+                ox[i] = (double)((t % c->mx1) * TILE);
+                oy[i] = (double)((t / c->mx1) * TILE);
+            }
+    }
     for (int k = 0; k < 8; ++k) {
         QP_HIP(hipMemcpyAsync(h.data(), sp->a[k].p, h.size() * sizeof(R), hipMemcpyDeviceToHost,
                               c->stream));
         QP_HIP(hipStreamSynchronize(c->stream));
         // words 6 and 7 of an FP32 record are held as deviations from 1
         const double base = (P67Dev<R>::on && (k == 5 || k == 6)) ? 1.0 : 0.0;
-        for (int64_t i = 0; i < sp->np; ++i) part[k + 8 * i] = (double)h[i] + base;
+        if (TileRel<R>::on && k < 2) {
+            const std::vector<double> &o = k == 0 ? ox : oy;
+            for (int64_t i = 0; i < sp->np; ++i) part[k + 8 * i] = (double)h[i] + o[i];
+        } else {
+            for (int64_t i = 0; i < sp->np; ++i) part[k + 8 * i] = (double)h[i] + base;
+        }
     }
     (void)ld;
     return QP_OK;
@@ -1980,9 +2094,27 @@ static int upload_t(qp_species *sp, const double *part, int64_t np)
 {
     qp_ctx *c = sp->ctx;
     std::vector<R> h((size_t)np);
+    // FP32: cell keys from the positions in grid units, and the positions
+    // relative to their tile (in FP64, then rounded)
+    std::vector<int> hkeys;
+    if (TileRel<R>::on) {
+        hkeys.resize((size_t)np);
+        for (int64_t i = 0; i < np; ++i) {
+            const double x = part[8 * i], y = part[1 + 8 * i];
+            QP_REQUIRE(x >= 0.0 && x < (double)c->nx && y >= 0.0 && y < (double)c->ny,
+                       QP_ERR_CHECK, "particle load: particle position outside the grid");
+            hkeys[i] = host_cell_key((int)x, (int)y, c->mx1);
+        }
+    }
     for (int k = 0; k < 8; ++k) {
         const double base = (P67Dev<R>::on && (k == 5 || k == 6)) ? 1.0 : 0.0;
-        for (int64_t i = 0; i < np; ++i) h[i] = (R)(part[k + 8 * i] - base);
+        if (TileRel<R>::on && k < 2)
+            for (int64_t i = 0; i < np; ++i) {
+                const double v = part[k + 8 * i];
+                h[i] = (R)(v - (double)((int)v & ~(TILE - 1)));
+            }
+        else
+            for (int64_t i = 0; i < np; ++i) h[i] = (R)(part[k + 8 * i] - base);
         QP_HIP(hipMemcpyAsync(sp->a[k].p, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice,
                               c->stream));
         QP_HIP(hipStreamSynchronize(c->stream));
@@ -1992,7 +2124,7 @@ static int upload_t(qp_species *sp, const double *part, int64_t np)
     bool uniform = true;
     const R q0 = np ? (R)part[7] : R(0);
     for (int64_t i = 1; uniform && i < np; ++i) uniform = (R)part[7 + 8 * i] == q0;
-    int rc = rebin_unsorted<R>(sp);
+    int rc = rebin_unsorted<R>(sp, TileRel<R>::on ? &hkeys : nullptr);
     if (rc) return rc;
     return set_q_uniform(sp, uniform, (double)q0);
 }

@@ -4,7 +4,8 @@ short 3-D step, against the oracle's restatement of the same sequence
 
 FP64: fields and particle records within 1e-10 relative after each of the
 first slices; per-cell particle counts bit-exact.  FP32: the same quantities
-within 1e-4 relative of the FP64 oracle after a few slices.
+within 1e-5 relative of the FP64 oracle (the precision form's contract),
+plane by plane after 4 slices and against each field's peak over 16.
 """
 import numpy as np
 import pytest
@@ -80,21 +81,38 @@ def test_slices_fp64():
 
 
 def test_slices_fp32_vs_fp64_oracle():
+    """FP32 at the contract bar, 1e-5: every plane after 4 slices relative
+    to its own peak, and the stored wake of 16 slices (Ez, Ex - By, psi)
+    relative to each field's peak over them.  Bz, the curl of the plasma's
+    transverse current (a small remainder of currents that nearly cancel
+    around the axis), is held to 1e-4 of its own peak, about 1e-7 of B_perp's."""
     banner("slices fp32")
     ctx, osim, gsim, sp = make_pair(qps.FP32)
     osim.begin_step()
     gsim.begin_step()
-    for j in range(1, 5):
+    got, want = {}, {}
+    for j in range(1, 17):
         osim.slice(j)
         gsim.slice(j)
-    compare_state(osim, gsim, sp, 4, 1e-4)
+        if j == 4:
+            compare_state(osim, gsim, sp, 4, 1e-5)
+        e3, b3 = gsim.bexyz, gsim.bbxyz
+        got[j] = {"Ez": e3.plane(3, j + 1), "Ex-By": e3.plane(1, j + 1) - b3.plane(2, j + 1),
+                  "Bz": b3.plane(3, j + 1), "psi": gsim.field(qps.F_PSI).download(0)[0]}
+        want[j] = {"Ez": osim.e3[2, j], "Ex-By": osim.e3[0, j] - osim.b3[1, j],
+                   "Bz": osim.b3[2, j], "psi": osim.psi[0]}
+    for k in ("Ez", "Ex-By", "Bz", "psi"):
+        peak = max(float(np.abs(want[j][k]).max()) for j in want)
+        err = max(float(np.abs(got[j][k] - want[j][k]).max()) for j in want) / peak
+        print(f"fp32 over 16 slices: {k} {err:.2e} of its peak", flush=True)
+        assert err < (1e-4 if k == "Bz" else 1e-5), (k, err)
     ctx.close()
 
 
 @pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
 def test_every_fp32_deposit_shape_matches_oracle(knob):
     """The FP32 launch shapes (their own table) against the FP64 oracle at
-    the FP32 bar, 1e-4 relative after 4 slices."""
+    the FP32 bar, 1e-5 relative after 4 slices."""
     banner(f"fp32 deposit shapes {knob}")
     ctx0 = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP32)
     _, nv = ctx0.tune_get(getattr(qps, knob))
@@ -111,7 +129,7 @@ def test_every_fp32_deposit_shape_matches_oracle(knob):
         gsim.begin_step()
         for j in range(1, 5):
             gsim.slice(j)
-        compare_state(osim, gsim, sp, 4, 1e-4)
+        compare_state(osim, gsim, sp, 4, 1e-5)
         ctx.close()
 
 
