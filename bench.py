@@ -5,14 +5,20 @@ x, y in [-10, 10] c/wp, xi in [0, 10] c/wp with 1024 slices, uniform plasma with
 4 x 4 particles per cell, a bi-Gaussian electron driver of 2^25 particles
 (nb/n0 = 10, sigma_r = 0.3, sigma_z = 0.5, gamma = 2e4, centred at xi = 2).
 
-A step is one whole 3-D step of the reference's loop (simulation_class.f03:
-611-765): the beam charge deposit, all 1024 xi slices (beam-charge copy,
-charge deposit, psi, E_perp, the B_perp predictor-corrector to relative change
-1e-6 with at most 20 passes, the plasma push with its cell sort, the slice
-writes into the 3-D wake), the beam push and the plasma renewal.  `value` is
-1024 slices x K steps / the timed wall clock, so the beam kernels are
-amortised in it.  Inputs are resident on the GPU when timing starts; the clock
-is bracketed by stream synchronisations.
+A step is one 3-D step of the reference's loop (simulation_class.f03:
+611-765): the beam charge deposit, the xi slices (beam-charge copy, charge
+deposit, psi, E_perp, the B_perp predictor-corrector to relative change 1e-6
+with at most 20 passes, the plasma push with its cell sort, the slice writes
+into the 3-D wake), the beam push and the plasma renewal.  The slices are all
+1024 where the reference completes the step.  Behind a strong driver a plasma
+electron eventually crosses more than one tile in one slice (the quasi-static
+push's dx/dxi = p_perp / (1 + psi) grows without bound as psi approaches -1),
+and the reference stops there (PPPCHECK2L, part2d_class.f03:582-588); an
+untimed probe step with the strict move check finds that slice for each
+precision, and the step then covers the slices before the earlier of the two
+stops.  `value` is those slices x K steps / the timed wall clock, so the beam
+kernels are amortised in it.  Inputs are resident on the GPU when timing
+starts; the clock is bracketed by stream synchronisations.
 
 Legs: the FP32 precision form (the headline, nb/n0 = 10) and the plain FP64
 path on the same beam, each also at nb/n0 = 1 and 20.  The first warm-up step
@@ -37,8 +43,9 @@ for _p in (REPO, PKG):
 METRIC = ("ξ-slices/s at 1024², 16 ppc; CNN-seeded FP32 vs plain FP64 at same B "
           "residual")
 HBM_PEAK_GBS = 8000.0
-PROFILES = {"float": os.path.join(REPO, "profiles", "step_fp32_1024sq_16ppc_nb10.md"),
-            "double": os.path.join(REPO, "profiles", "step_fp64_1024sq_16ppc_nb10.md")}
+# one profiled run of the bench command holds both legs' kernels
+PROFILES = {p: os.path.join(REPO, "profiles", "step_1024sq_16ppc_nb10.md")
+            for p in ("float", "double")}
 KERNEL_SYMBOL = {"amjdp": "k_amjdp<{p}", "push": "k_push<{p},1>", "extpsi": "k_extpsi<{p}",
                  "deposit_q": "k_deposit_q<{p}", "sort": "k_push<{p},2>"}
 # slices whose wake planes the first step of each leg keeps (FP32 vs FP64)
@@ -71,14 +78,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_step(sim, args, samples=None, psi=None):
-    """One 3-D step; returns the corrector passes of every slice.  With
-    samples, psi[j] is downloaded after each sampled slice j (untimed steps
-    only)."""
+def run_step(sim, args, nslices, samples=None, psi=None):
+    """One 3-D step over slices 1..nslices; returns the corrector passes of
+    every slice.  With samples, psi[j] is downloaded after each sampled
+    slice j (untimed steps only)."""
     import qpslice as qps
     sim.begin_step()
     passes = []
-    for j in range(1, sim.ctx.nz + 1):
+    for j in range(1, nslices + 1):
         it, _ = sim.slice(j, iter_max=args.iter_max, tol=args.tol)
         passes.append(it)
         if samples and j in samples:
@@ -87,9 +94,10 @@ def run_step(sim, args, samples=None, psi=None):
     return passes
 
 
-def run_leg(args, precision, peak, nwarm, nsteps, beam_part=None, breakdown=False):
+def run_leg(args, precision, peak, nwarm, nsteps, nslices, beam_part=None, breakdown=False):
     """One context: W warm-up steps (the first samples the wake), K timed
-    steps, optionally one more step with every kernel class timed."""
+    steps of nslices slices, optionally one more step with every kernel
+    class timed."""
     import numpy as np
 
     import qpslice as qps
@@ -115,7 +123,7 @@ def run_leg(args, precision, peak, nwarm, nsteps, beam_part=None, breakdown=Fals
         part = bm.download() if beam_part is None else None
         log(f"[bench] {name} nb/n0={peak:g}: set-up {time.time() - t0:.1f}s "
             f"(plasma {sp.np}, beam {bm.np})")
-        nz = ctx.nz
+        nz = nslices
         samples = [j for j in SAMPLES if j <= nz]
         wake, energy, first_passes = None, None, None
         for w in range(nwarm):
@@ -123,14 +131,14 @@ def run_leg(args, precision, peak, nwarm, nsteps, beam_part=None, breakdown=Fals
             if w == 0:
                 psi = {}
                 g0 = bm.moments()[0]
-                first_passes = run_step(sim, args, samples, psi)
+                first_passes = run_step(sim, args, nz, samples, psi)
                 e3, b3 = sim.bexyz, sim.bbxyz
                 wake = {j: {"Ez": e3.plane(3, j + 1), "Ex-By": e3.plane(1, j + 1) - b3.plane(2, j + 1),
                             "Ey+Bx": e3.plane(2, j + 1) + b3.plane(1, j + 1),
                             "Bz": b3.plane(3, j + 1), "psi": psi[j]} for j in samples}
                 energy = (g0, bm.moments()[0])
             else:
-                run_step(sim, args)
+                run_step(sim, args, nz)
             log(f"[bench] {name} nb/n0={peak:g}: warm-up step {w + 1} {time.time() - t0:.2f}s")
         # timed steps: HIP events on the current deposit's launches only
         ctx.profile(True, classes=[qps.PROF_AMJDP])
@@ -138,7 +146,7 @@ def run_leg(args, precision, peak, nwarm, nsteps, beam_part=None, breakdown=Fals
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(nsteps):
-            passes += run_step(sim, args)
+            passes += run_step(sim, args, nz)
         ctx.sync()
         elapsed = time.perf_counter() - t0
         am_ms, am_n, am_b = ctx.profile_get(qps.PROF_AMJDP)
@@ -165,7 +173,7 @@ def run_leg(args, precision, peak, nwarm, nsteps, beam_part=None, breakdown=Fals
             # every kernel class over one more step (events on every launch;
             # not part of the timed value)
             ctx.profile(True)
-            bp = run_step(sim, args)
+            bp = run_step(sim, args, nz)
             ctx.sync()
             classes = {}
             total = 0.0
@@ -355,45 +363,65 @@ def cpu_baseline(args, passes):
     return info
 
 
-def probe(args):
-    """One untimed 3-D step per precision and nb/n0 with the strict move
-    check: where the step stops (a particle crossing more than one tile in a
-    slice, PPPCHECK2L), and the corrector passes per slice."""
+def probe_leg(args, pk, prec):
+    """One untimed 3-D step with the strict move check: the slices computed
+    before the step stops (a particle crossing more than one tile in a
+    slice, PPPCHECK2L, where the reference stops), the reason, and the
+    corrector passes per slice."""
     import numpy as np
 
+    import qpslice as qps
+    name = "fp32" if prec == qps.FP32 else "fp64"
+    ctx = qps.Ctx(args.ind, args.ind, args.indz, precision=prec)
+    try:
+        ctx.tune(qps.TUNE_STRICT, 1)
+        ctx.tune(qps.TUNE_SEED, args.fp32_seed if prec == qps.FP32 else 0)
+        sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
+        bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0), sigma=(args.sigma_r, args.sigma_r, 0.5),
+                      sigma_v=(1.0, 1.0, 0.0), np_=tuple(args.beam_np), gamma=2.0e4, peak=pk,
+                      dt=10.0)
+        sim = qps.Sim(ctx, [sp], [bm], iter_=2)
+        sim.begin_step()
+        passes, stop = [], None
+        t0 = time.time()
+        try:
+            for j in range(1, ctx.nz + 1):
+                passes.append(sim.slice(j, iter_max=args.iter_max, tol=args.tol)[0])
+            sim.end_step()
+        except qps.QPError as e:
+            stop = str(e)
+        p = np.array(passes)
+        out = {"slices_done": len(passes), "stop": stop, "seconds": round(time.time() - t0, 2),
+               "passes_mean": float(p.mean()) if len(p) else None,
+               "passes_max": int(p.max()) if len(p) else None,
+               "passes_per_128": [round(float(p[i:i + 128].mean()), 2)
+                                  for i in range(0, len(p), 128)]}
+        log(f"[bench] probe nb/n0={pk:g} {name}: {out}")
+        return out
+    finally:
+        ctx.close()
+
+
+def probe(args):
+    """One strict-checked step per precision and nb/n0 (probe_leg)."""
     import qpslice as qps
     out = {}
     for pk in [args.peak] + list(args.other_peaks):
         for prec, name in ((qps.FP32, "fp32"), (qps.FP64, "fp64")):
-            ctx = qps.Ctx(args.ind, args.ind, args.indz, precision=prec)
-            try:
-                ctx.tune(qps.TUNE_STRICT, 1)
-                ctx.tune(qps.TUNE_SEED, args.fp32_seed if prec == qps.FP32 else 0)
-                sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
-                bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0),
-                              sigma=(args.sigma_r, args.sigma_r, 0.5), sigma_v=(1.0, 1.0, 0.0),
-                              np_=tuple(args.beam_np), gamma=2.0e4, peak=pk, dt=10.0)
-                sim = qps.Sim(ctx, [sp], [bm], iter_=2)
-                sim.begin_step()
-                passes, stop = [], None
-                t0 = time.time()
-                try:
-                    for j in range(1, ctx.nz + 1):
-                        passes.append(sim.slice(j, iter_max=args.iter_max, tol=args.tol)[0])
-                    sim.end_step()
-                except qps.QPError as e:
-                    stop = str(e)
-                p = np.array(passes)
-                out[f"nb/n0={pk:g} {name}"] = {
-                    "slices_done": len(passes), "stop": stop, "seconds": round(time.time() - t0, 2),
-                    "passes_mean": float(p.mean()) if len(p) else None,
-                    "passes_max": int(p.max()) if len(p) else None,
-                    "passes_per_128": [round(float(p[i:i + 128].mean()), 2)
-                                       for i in range(0, len(p), 128)]}
-                log(f"[bench] probe nb/n0={pk:g} {name}: {out[f'nb/n0={pk:g} {name}']}")
-            finally:
-                ctx.close()
+            out[f"nb/n0={pk:g} {name}"] = probe_leg(args, pk, prec)
     print(json.dumps(out), flush=True)
+
+
+def leg_slices(args, pk):
+    """The slices a step of this nb/n0 covers: all of them if neither
+    precision's probe step stops, else those before the earlier stop."""
+    import qpslice as qps
+    res = {"fp32": probe_leg(args, pk, qps.FP32)}
+    if args.fp64:
+        res["fp64"] = probe_leg(args, pk, qps.FP64)
+    n = min(r["slices_done"] for r in res.values())
+    stops = {k: r["stop"] for k, r in res.items() if r["stop"]}
+    return n, stops
 
 
 def main():
@@ -413,7 +441,7 @@ def main():
                     help="corrector seed of the FP32 leg: 0 previous slice, 1 extrapolated")
     ap.add_argument("--fp32-mix", type=int, default=0, choices=(0, 1),
                     help="corrector iteration of the FP32 leg: 0 plain, 1 Anderson-mixed")
-    ap.add_argument("--speculate", type=int, default=1, choices=(0, 1),
+    ap.add_argument("--speculate", type=int, default=1, choices=(0, 1, 2),
                     help="enqueue the next corrector pass's deposits before reading a residual")
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
@@ -441,22 +469,26 @@ def main():
         probe(args)
         return
     import qpslice as qps
-    legs = {}
-    r32, beam = run_leg(args, qps.FP32, args.peak, args.warmup, args.steps,
+    legs, span = {}, {}
+    for pk in [args.peak] + [p for p in args.other_peaks if p != args.peak]:
+        span[pk] = leg_slices(args, pk)
+    n0 = span[args.peak][0]
+    r32, beam = run_leg(args, qps.FP32, args.peak, args.warmup, args.steps, n0,
                         breakdown=args.breakdown)
     legs[args.peak] = {"fp32": r32}
     if args.fp64:
         r64, _ = run_leg(args, qps.FP64, args.peak, max(1, min(args.warmup, 1)),
-                         max(1, min(args.steps, 1)), beam_part=beam, breakdown=args.breakdown)
+                         max(1, min(args.steps, 1)), n0, beam_part=beam,
+                         breakdown=args.breakdown)
         legs[args.peak]["fp64"] = r64
     del beam
     for pk in args.other_peaks:
         if pk == args.peak:
             continue
-        a, b = run_leg(args, qps.FP32, pk, 1, 1)
+        a, b = run_leg(args, qps.FP32, pk, 1, 1, span[pk][0])
         legs[pk] = {"fp32": a}
         if args.fp64:
-            legs[pk]["fp64"], _ = run_leg(args, qps.FP64, pk, 1, 1, beam_part=b)
+            legs[pk]["fp64"], _ = run_leg(args, qps.FP64, pk, 1, 1, span[pk][0], beam_part=b)
     for pk, lg in legs.items():
         if "fp64" in lg:
             lg["err"] = wake_errors(lg["fp32"], lg["fp64"])
@@ -483,9 +515,14 @@ def main():
         "config": {
             "workload": f"precision form: {2 ** args.ind}^2 x {2 ** args.indz} slices, "
                         f"{args.ppc * args.ppc} ppc, beam {r32['np_beam']} particles, "
-                        f"nb/n0={args.peak:g}, B residual {args.tol:g} (max {args.iter_max} "
-                        "passes); a step is one whole 3-D step (beam deposit, every slice, "
-                        "beam push, plasma renewal); FP32 corrector seed: "
+                        f"nb/n0={args.peak:g}, sigma_r={args.sigma_r:g}, B residual {args.tol:g} "
+                        f"(max {args.iter_max} passes); a step is one 3-D step (beam deposit, "
+                        f"slices 1..{n0}, beam push, plasma renewal): "
+                        + ("every slice" if n0 == 2 ** args.indz else
+                           f"the slices before the reference's move check (PPPCHECK2L) stops "
+                           f"the step at slice {n0 + 1} (a plasma electron crossing more than "
+                           "one tile)")
+                        + "; FP32 corrector seed: "
                         + {0: "previous slice", 1: "linear extrapolation"}[args.fp32_seed]
                         + {0: "", 1: ", Anderson-mixed passes"}[args.fp32_mix]
                         + "; FP64 leg: previous slice (plain)",
@@ -496,7 +533,8 @@ def main():
             "ppc": args.ppc * args.ppc,
             "plasma_particles": r32["np_plasma"],
             "beam_particles": r32["np_beam"],
-            "slices_per_step": 2 ** args.indz,
+            "slices_per_step": n0,
+            "step_stop": span[args.peak][1] or None,
             "passes_per_slice_mean": round(r32["passes_mean"], 3),
             "passes_per_slice_max": r32["passes_max"],
         },
@@ -515,7 +553,7 @@ def main():
                              "kernels": r64.get("kernels")}
     out["legs"] = {}
     for pk, lg in legs.items():
-        e = {}
+        e = {"slices_per_step": span[pk][0], "step_stop": span[pk][1] or None}
         for prec in ("fp32", "fp64"):
             if prec in lg:
                 r = lg[prec]

@@ -108,7 +108,11 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * differ only in how a cell's particles are split over lanes, so results
  * agree to rounding.  QP_TUNE_SPECULATE = 1 lets qp_slice_run enqueue the
  * deposits of corrector pass l+1 before reading the residual of pass l (they
- * return at once if pass l converged; the default); 0 waits for each residual.
+ * return at once if pass l converged; the default); 0 waits for each residual;
+ * 2 also enqueues as many whole passes as the last slice used before the
+ * first residual read (the passes after a converged one return at once), so
+ * a slice that converges within them costs one host wait.  The passes and
+ * results are the same in every mode.
  * QP_TUNE_SEED = 1 seeds the corrector of a slice run
  * with a residual tolerance by linear extrapolation 2 B(xi-1) - B(xi-2)
  * instead of the reference's B(xi-1) (default 0); QP_TUNE_SEED = 2 keeps the
@@ -233,6 +237,24 @@ int qp_species_renew(qp_species *sp, double s);                   /* :138 */
  * decrease. */
 int qp_species_set_longitudinal(qp_species *sp, int n, const double *s_pts,
                                 const double *fs, double s_now);
+/* Transverse profiles other than uniform (profile 0): 10, an elliptical
+ * annulus (dist2d_010, fdist2d_class.f03:318-523); 11, a Gaussian ring,
+ * q weighted by exp(-(r - radius)^2 / (2 width^2)) (dist2d_011, :525-711);
+ * 12, a piecewise-linear radial density fr(r) (dist2d_012, :713-950).
+ * Lengths in c/wp, center in box coordinates; the longitudinal profile
+ * still scales every reload.  Reloads the plasma at s_now.  QP_ERR_ARG
+ * with the reference's message when the radial points do not increase. */
+typedef struct {
+    int profile;               /* 0, 10, 11 or 12 */
+    double center[2];          /* 10, 11, 12 */
+    double inner_radius[2];    /* 10: x and y semi-axes */
+    double outer_radius[2];    /* 10 */
+    double radius, width;      /* 11 */
+    int nr;                    /* 12: number of radial points */
+    const double *r;           /* 12: radii */
+    const double *fr;          /* 12: relative densities at r */
+} qp_species_profile;
+int qp_species_set_profile(qp_species *sp, const qp_species_profile *p, double s_now);
 int qp_species_qdp(qp_species *sp, qp_field2d *q);                /* :166 */
 /* qp_species_qdp without any allocation.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory (code under HIP graph
@@ -301,11 +323,45 @@ typedef struct {
     int64_t npmax;
 } qp_beam_twiss_params;
 
-/* init_beam3d :61-93; seed 0 is the reference's fixed ranorm state,
- * other seeds start independent streams.  np = {0,0,0} with npmax > 0 makes
- * an empty beam of that capacity, to be filled by qp_beam_upload. */
+/* init_beam3d :61-93; seed 0 continues the context's ranorm stream, which
+ * starts at the reference's DATA state (ranorm's SAVE'd state, shared by
+ * every beam the reference loads); other seeds start independent streams.
+ * np = {0,0,0} with npmax > 0 makes an empty beam of that capacity, to be
+ * filled by qp_beam_upload. */
 int qp_beam_create(qp_ctx *ctx, const qp_beam_params *p, double dt,
                    uint64_t seed, qp_beam **out);
+/* Beam profile 3 (init_fdist3d_003 + dist3d_003, fdist3d_class.f03:
+ * 823-974): np records "z x y vz vx vy q" read from a text file
+ * (list-directed), positions in units of 5.32150254e9/sqrt(n0) converted to
+ * grid units (x, y about center, z = -z), kept inside the grid.  QP_ERR_ARG
+ * "Read beam file error" when the file holds fewer than np records. */
+typedef struct {
+    const char *file_name;
+    double center[3];      /* c/wp, box coordinates (x and y used) */
+    double n0;             /* simulation.n0 */
+    double q, m;
+    int np;                /* records to read */
+    int evolution;
+    int64_t npmax;         /* 0: np + 2 */
+} qp_beam_file_params;
+int qp_beam_create_file(qp_ctx *ctx, const qp_beam_file_params *p, double dt,
+                        qp_beam **out);
+/* Beam profile 100 (init_fdist3d_100 + dist3d_100, fdist3d_class.f03:
+ * 976-1166): a quiet-start lattice of ppc(1:3) particles per cell over the
+ * range box, p = (0, 0, gamma), charge sign(q)/(ppc product) * np *
+ * exp(-((z - zc)^2 + (|r_perp - c| - r1)^2) / (2 r2^2)) in cell units
+ * (r1, r2 in x cells): a Gaussian torus. */
+typedef struct {
+    double center[3];      /* c/wp, box coordinates */
+    double range_x[2], range_y[2], range_z[2];   /* c/wp */
+    int ppc[3];
+    double q, m, np, gamma;
+    double r1, r2;         /* c/wp */
+    int evolution;
+    int64_t npmax;         /* 0: the lattice's count + 2 */
+} qp_beam_torus_params;
+int qp_beam_create_torus(qp_ctx *ctx, const qp_beam_torus_params *p, double dt,
+                         qp_beam **out);
 /* init_fdist3d_002 + dist3d_002 (fdist3d_class.f03:652-821); seeds as above */
 int qp_beam_create_twiss(qp_ctx *ctx, const qp_beam_twiss_params *p, double dt,
                          uint64_t seed, qp_beam **out);

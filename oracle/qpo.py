@@ -198,13 +198,23 @@ class TwissSpec:
 
 @dataclass
 class SpeciesSpec:
-    """Uniform plasma, profile 0 (fdist2d_class.f03:168-316); piecewise: an
-    optional longitudinal profile (s points, fs factors) of :190-201."""
+    """A plasma species: uniform (profile 0, fdist2d_class.f03:168-316), an
+    elliptical annulus (10, :318-523), a Gaussian ring (11, :525-711) or a
+    piecewise-linear radial density (12, :713-950), centred at `center`
+    (c/wp, box coordinates); piecewise: an optional longitudinal profile (s
+    points, fs factors) of :190-201, for every profile."""
     ppc: tuple = (2, 2)
     density: float = 1.0
     q: float = -1.0
     m: float = 1.0
     piecewise: tuple = None
+    profile: int = 0
+    center: tuple = (0.0, 0.0)
+    inner_radius: tuple = (0.0, 0.0)   # profile 10
+    outer_radius: tuple = (0.0, 0.0)   # profile 10
+    radius: float = 0.0                # profile 11
+    width: float = 1.0                 # profile 11
+    radial: tuple = None               # profile 12: (r points, densities)
 
 
 def density_factor(piecewise, s):
@@ -319,6 +329,72 @@ def beam_init_twiss(grid: Grid, b: TwissSpec, npmax=None, rng=None):
     return np.array(out, dtype=np.float64).reshape(-1, 7), qm, ierr
 
 
+def beam_init_file(grid: Grid, path, np_, center=(0.0, 0.0, 0.0), n0=1e16):
+    """dist3d_003 (fdist3d_class.f03:899-974, one partition): np records
+    "z x y vz vx vy q" of a text file (blanks or commas, D exponents),
+    tx = x/cwp/dx + x0, ty = y/cwp/dy + y0, tz = -z/cwp/dz with cwp =
+    5.32150254e9/sqrt(n0) (:846) and x0, y0 the centre in grid units; kept
+    when 1 <= tx <= nx-1, 0 <= ty < ny, 0 <= tz < nz.  Returns (n, 7)."""
+    cwp = 5.32150254 * 1e9 / np.sqrt(n0)
+    x0 = (center[0] - grid.box[0][0]) / grid.dx
+    y0 = (center[1] - grid.box[1][0]) / grid.dy
+    with open(path) as fh:
+        vals = [float(t.replace("D", "e").replace("d", "e"))
+                for t in fh.read().replace(",", " ").split()]
+    if len(vals) < 7 * np_:
+        raise ValueError("Read beam file error")
+    rec = np.array(vals[:7 * np_]).reshape(np_, 7)
+    tx = rec[:, 1] / cwp / grid.dx + x0
+    ty = rec[:, 2] / cwp / grid.dy + y0
+    tz = -rec[:, 0] / cwp / grid.dz
+    keep = (tx >= 1) & (tx <= grid.nx - 1) & (ty >= 0) & (ty < grid.ny) & (tz >= 0) & \
+        (tz < grid.nz)
+    out = np.stack([tx, ty, tz, rec[:, 4], rec[:, 5], rec[:, 3], rec[:, 6]], axis=1)
+    return np.ascontiguousarray(out[keep])
+
+
+def beam_init_torus(grid: Grid, center, range_x, range_y, range_z, ppc, q, np_, gamma, r1, r2):
+    """dist3d_100 (fdist3d_class.f03:1084-1166, one partition): the lattice
+    (ix + 0.5)/xppc + i - 1 over int(lb)..int(ub)-1 of the range box (grid
+    units, clipped to the grid), p = (0, 0, gamma), q = qm np exp(-a4/r2^2
+    * 0.5) with a4 = (tz - bcz)^2 + (sqrt((tx-bcx)^2 + (ty-bcy)^2) - r1)^2,
+    qm = sign(q)/(xppc yppc zppc) and r1, r2 in x cells (:1041-1076).
+    Returns (n, 7)."""
+    d = (grid.dx, grid.dy, grid.dz)
+    mins = (grid.box[0][0], grid.box[1][0], grid.box[2][0])
+    nd = (grid.nx, grid.ny, grid.nz)
+    bc = [(center[k] - mins[k]) / d[k] for k in range(3)]
+    rng_ = (range_x, range_y, range_z)
+    lb = [(rng_[k][0] - mins[k]) / d[k] for k in range(3)]
+    ub = [(rng_[k][1] - mins[k]) / d[k] for k in range(3)]
+    for k in (1, 2):
+        if lb[k] > nd[k] or ub[k] < 0:
+            return np.zeros((0, 7))
+    lb = [max(lb[k], 0.0) for k in range(3)]
+    ub = [min(ub[k], float(nd[k])) for k in range(3)]
+    qm = q / abs(q) / ppc[0] / ppc[1] / ppc[2]
+    r1g, r2g = r1 / grid.dx, r2 / grid.dx
+    axes = []
+    for k in range(3):
+        cells = np.arange(int(lb[k]), int(ub[k]))
+        off = (np.arange(ppc[k]) + 0.5) / ppc[k]
+        axes.append((off[None, :] + cells[:, None] - 1))
+    rows = []
+    for i in range(axes[0].shape[0]):
+        for j in range(axes[1].shape[0]):
+            for k in range(axes[2].shape[0]):
+                tx, ty, tz = np.meshgrid(axes[0][i], axes[1][j], axes[2][k], indexing="ij")
+                tx, ty, tz = tx.ravel(), ty.ravel(), tz.ravel()
+                a1 = np.abs(tz - bc[2])
+                a2 = np.sqrt((tx - bc[0]) ** 2 + (ty - bc[1]) ** 2)
+                a3 = np.abs(a2 - r1g)
+                a4 = a1 ** 2 + a3 ** 2
+                qq = qm * np_ * np.exp(-a4 / r2g ** 2 * 0.5)
+                z0 = np.zeros_like(tx)
+                rows.append(np.stack([tx, ty, tz, z0, z0, z0 + gamma, qq], axis=1))
+    return np.concatenate(rows) if rows else np.zeros((0, 7))
+
+
 def beam_init(grid: Grid, b: BeamSpec, npmax=None, rng=None):
     """Returns (particles (np,7), qm, ierr)."""
     npx, npy, npz = b.np_
@@ -339,6 +415,57 @@ def plasma_init(nx, ny, xppc, yppc, den=1.0, q=-1.0):
     got = lib().qpo_plasma_init(nx, ny, xppc, yppc, den, q, _p(part))
     assert got == n
     return part
+
+
+def plasma_init_profile(grid, spec, den):
+    """dist2d_010 / dist2d_011 / dist2d_012 with one partition (noff = 0):
+    the quiet-start lattice of dist2d_000, then the profile's selection and
+    charge weight in the reference's operation order (numpy FP64):
+      10 keeps a particle unless (x-cx)^2/irx^2 + (y-cy)^2/iry^2 < 1 or
+         (x-cx)^2/orx^2 + (y-cy)^2/ory^2 > 1 (fdist2d_class.f03:438-441);
+      11 weights q by exp(-(r-r0)^2 * 0.5/sigr^2) (:633-645);
+      12 keeps r(1) <= r <= r(n) and weights q by fr linearly interpolated
+         at r (:836-856), checking that r increases.
+    Radii are in grid units (divided by dx; profile 10's y radii by dy) and
+    the centre is (center - box min) / dx (:340-385, 547-588, 740-775)."""
+    nx, ny = grid.nx, grid.ny
+    part = plasma_init(nx, ny, spec.ppc[0], spec.ppc[1], den, spec.q)
+    dx, dy = grid.dx, grid.dy
+    cx = (spec.center[0] - grid.box[0][0]) / dx
+    cy = (spec.center[1] - grid.box[1][0]) / dy
+    x, y = part[:, 0], part[:, 1]
+    if spec.profile == 10:
+        irx, iry = spec.inner_radius[0] / dx, spec.inner_radius[1] / dy
+        orx, ory = spec.outer_radius[0] / dx, spec.outer_radius[1] / dy
+        iirx2, iiry2 = 1.0 / irx ** 2, 1.0 / iry ** 2
+        iorx2, iory2 = 1.0 / orx ** 2, 1.0 / ory ** 2
+        drop = (((x - cx) ** 2 * iirx2 + (y - cy) ** 2 * iiry2) < 1) | \
+               (((x - cx) ** 2 * iorx2 + (y - cy) ** 2 * iory2) > 1)
+        return np.ascontiguousarray(part[~drop])
+    rr = np.sqrt((x - cx) ** 2 + (y - cy) ** 2)
+    if spec.profile == 11:
+        r0, sigr = spec.radius / dx, spec.width / dx
+        isigr2 = 0.5 / sigr ** 2
+        part[:, 7] = part[:, 7] * np.exp(-(rr - r0) ** 2 * isigr2)
+        return part
+    if spec.profile == 12:
+        rp = np.asarray(spec.radial[0], dtype=np.float64) / dx
+        fr = np.asarray(spec.radial[1], dtype=np.float64)
+        if len(rp) != len(fr):
+            raise ValueError("The piecewise_radial_density and r array have different sizes!")
+        keep = ~((rr < rp[0]) | (rr > rp[-1]))
+        w = np.zeros_like(rr)
+        for k in np.nonzero(keep)[0]:
+            for ii in range(1, len(rp)):
+                if rp[ii] <= rp[ii - 1]:
+                    raise ValueError("r is not monotonically increasing!")
+                if rr[k] <= rp[ii]:
+                    w[k] = fr[ii - 1] + (fr[ii] - fr[ii - 1]) / (rp[ii] - rp[ii - 1]) * \
+                        (rr[k] - rp[ii - 1])
+                    break
+        part[:, 7] = part[:, 7] * w
+        return np.ascontiguousarray(part[keep])
+    raise ValueError(f"plasma profile {spec.profile} is not one of 0, 10, 11, 12")
 
 
 # --------------------------------------------------------------------------
@@ -530,8 +657,11 @@ class Species:
         density first, as den_temp*den in dist2d_000 (:254)."""
         g = self.grid
         den = density_factor(self.spec.piecewise, s) * self.spec.density
-        self.part = plasma_init(g.nx, g.ny, self.spec.ppc[0], self.spec.ppc[1], den,
-                                self.spec.q)
+        if self.spec.profile == 0:
+            self.part = plasma_init(g.nx, g.ny, self.spec.ppc[0], self.spec.ppc[1], den,
+                                    self.spec.q)
+        else:
+            self.part = plasma_init_profile(g, self.spec, den)
         self.kpic = order_by_tile(g.nx, g.ny, self.part)
         qn = np.zeros((1, g.ny + 1, g.nx + 1))
         qdeposit(g.nx, g.ny, self.part, qn)

@@ -8,8 +8,12 @@
 // when a slice is copied out (qdpcopy).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "qp_common.h"
 
@@ -493,6 +497,146 @@ int qp_beam_create(qp_ctx *c, const qp_beam_params *p, double dt, uint64_t seed,
     if (rc) { qp_beam_destroy(b); return rc; }
     *out = b;
     return QP_OK;
+}
+
+// A beam whose particles the host has built (profiles 3 and 100): the
+// record of profile 0's parameter block that the push and diagnostics use
+static int beam_from_host(qp_ctx *c, double q, double m, int evolution, int64_t npmax,
+                          std::vector<double> (&h)[7], int64_t n, double dt, qp_beam **out)
+{
+    QP_REQUIRE(npmax < ((int64_t)1 << 31), QP_ERR_UNSUPPORTED, "beam: npmax too large");
+    QP_REQUIRE(n <= npmax, QP_ERR_ARG, "beam: %lld particles exceed npmax %lld", (long long)n,
+               (long long)npmax);
+    qp_beam_params bp{};
+    bp.q = q;
+    bp.m = m;
+    bp.evolution = evolution;
+    bp.npmax = npmax;
+    qp_beam *b = new qp_beam();
+    b->ctx = c;
+    b->p = bp;
+    b->qbm = q / m;
+    b->dt = dt;
+    b->npmax = npmax;
+    int rc = beam_alloc(b, npmax);
+    if (!rc) rc = beam_set(b, h, n);
+    if (rc) { qp_beam_destroy(b); return rc; }
+    *out = b;
+    return QP_OK;
+}
+
+// init_fdist3d_003 + dist3d_003 (fdist3d_class.f03:823-974): np records
+// "z x y vz vx vy q" of a text file (list-directed: blanks or commas between
+// values, Fortran D exponents), positions in units of cwp =
+// 5.32150254e9/sqrt(n0) converted to grid units, x and y about the centre,
+// z = -z/cwp/dz; kept if 1 <= x <= nx-1, 0 <= y < ny and 0 <= z < nz (one
+// partition); stored as x, y, z, vx, vy, vz, q.  A file with fewer than np
+// records fails with the reference's message.
+int qp_beam_create_file(qp_ctx *c, const qp_beam_file_params *p, double dt, qp_beam **out)
+{
+    QP_REQUIRE(c && p && out && p->file_name, QP_ERR_ARG, "qp_beam_create_file: null argument");
+    QP_REQUIRE(p->np >= 0 && p->n0 > 0.0 && p->q != 0.0 && p->m != 0.0, QP_ERR_ARG,
+               "qp_beam_create_file: bad beam parameters");
+    FILE *f = std::fopen(p->file_name, "r");
+    QP_REQUIRE(f, QP_ERR_ARG, "dist3d_003: cannot open %s", p->file_name);
+    const double cwp = 5.32150254 * 1e9 / std::sqrt(p->n0);
+    const double dx = c->dex, dy = c->dey, dz = c->dxi;
+    const double x0 = (p->center[0] - c->box[0]) / dx, y0 = (p->center[1] - c->box[2]) / dy;
+    const int64_t npmax = p->npmax > 0 ? p->npmax : (int64_t)p->np + 2;
+    std::vector<double> h[7];
+    for (auto &v : h) v.resize((size_t)npmax);
+    int64_t n = 0;
+    bool short_file = false;
+    std::string tok;
+    auto next = [&](double &v) -> bool {
+        tok.clear();
+        int ch;
+        while ((ch = std::fgetc(f)) != EOF && (std::isspace(ch) || ch == ',')) {}
+        if (ch == EOF) return false;
+        do {
+            tok.push_back((char)(ch == 'd' || ch == 'D' ? 'e' : ch));
+        } while ((ch = std::fgetc(f)) != EOF && !std::isspace(ch) && ch != ',');
+        char *end = nullptr;
+        v = std::strtod(tok.c_str(), &end);
+        return end && *end == '\0';
+    };
+    for (int i = 0; i < p->np && !short_file; ++i) {
+        double t[7];
+        for (int k = 0; k < 7; ++k)
+            if (!next(t[k])) { short_file = true; break; }
+        if (short_file) break;
+        const double tx = t[1] / cwp / dx + x0, ty = t[2] / cwp / dy + y0, tz = -t[0] / cwp / dz;
+        if (tx >= 1 && tx <= (double)(c->nx - 1) && ty >= 0.0 && ty < (double)c->ny && tz >= 0.0 &&
+            tz < (double)c->nz) {
+            if (n >= npmax) { std::fclose(f); QP_REQUIRE(false, QP_ERR_ARG, "dist3d_003: npmax %lld too small", (long long)npmax); }
+            h[0][n] = tx; h[1][n] = ty; h[2][n] = tz;
+            h[3][n] = t[4]; h[4][n] = t[5]; h[5][n] = t[3]; h[6][n] = t[6];
+            ++n;
+        }
+    }
+    std::fclose(f);
+    QP_REQUIRE(!short_file, QP_ERR_ARG, "dist3d_003: Read beam file error");
+    return beam_from_host(c, p->q, p->m, p->evolution, npmax, h, n, dt, out);
+}
+
+// init_fdist3d_100 + dist3d_100 (fdist3d_class.f03:976-1166): a quiet-start
+// lattice over the range box (clipped to the grid), tx = (ix + 0.5)/xppc +
+// i - 1 for i = int(lb)..int(ub)-1 (likewise y, z), momentum (0, 0, gamma),
+// charge qm np exp(-((tz - bcz)^2 + (|r - bc| - r1)^2) / r2^2 / 2) with
+// qm = sign(q)/(xppc yppc zppc), r1 and r2 in x-cells
+int qp_beam_create_torus(qp_ctx *c, const qp_beam_torus_params *p, double dt, qp_beam **out)
+{
+    QP_REQUIRE(c && p && out, QP_ERR_ARG, "qp_beam_create_torus: null argument");
+    QP_REQUIRE(p->ppc[0] > 0 && p->ppc[1] > 0 && p->ppc[2] > 0 && p->q != 0.0 && p->m != 0.0 &&
+                   p->r2 != 0.0,
+               QP_ERR_ARG, "qp_beam_create_torus: bad beam parameters");
+    const double dx = c->dex, dy = c->dey, dz = c->dxi;
+    const double bcx = (p->center[0] - c->box[0]) / dx, bcy = (p->center[1] - c->box[2]) / dy;
+    const double bcz = (p->center[2] - c->box[4]) / dz;
+    double lb[3] = {(p->range_x[0] - c->box[0]) / dx, (p->range_y[0] - c->box[2]) / dy,
+                    (p->range_z[0] - c->box[4]) / dz};
+    double ub[3] = {(p->range_x[1] - c->box[0]) / dx, (p->range_y[1] - c->box[2]) / dy,
+                    (p->range_z[1] - c->box[4]) / dz};
+    const double r1 = p->r1 / dx, r2 = p->r2 / dx;
+    const double qm = p->q / std::fabs(p->q) / p->ppc[0] / p->ppc[1] / p->ppc[2];
+    const int nd[3] = {c->nx, c->ny, c->nz};
+    std::vector<double> h[7];
+    int64_t n = 0;
+    bool outside = false;
+    for (int d = 0; d < 3; ++d) {
+        if (d > 0 && (lb[d] > (double)nd[d] || ub[d] < 0.0)) outside = true;   // :1108-1124
+        lb[d] = std::fmax(lb[d], 0.0);
+        ub[d] = std::fmin(ub[d], (double)nd[d]);
+    }
+    const int64_t cap = outside ? 0 : (int64_t)std::max(0, (int)ub[0] - (int)lb[0]) *
+                                          std::max(0, (int)ub[1] - (int)lb[1]) *
+                                          std::max(0, (int)ub[2] - (int)lb[2]) * p->ppc[0] *
+                                          p->ppc[1] * p->ppc[2];
+    const int64_t npmax = p->npmax > 0 ? p->npmax : cap + 2;
+    QP_REQUIRE(cap <= npmax, QP_ERR_ARG, "dist3d_100: %lld particles exceed npmax %lld",
+               (long long)cap, (long long)npmax);
+    for (auto &v : h) v.resize((size_t)npmax);
+    if (!outside)
+        for (int i = (int)lb[0]; i <= (int)ub[0] - 1; ++i)
+            for (int j = (int)lb[1]; j <= (int)ub[1] - 1; ++j)
+                for (int k = (int)lb[2]; k <= (int)ub[2] - 1; ++k)
+                    for (int ix = 0; ix < p->ppc[0]; ++ix)
+                        for (int iy = 0; iy < p->ppc[1]; ++iy)
+                            for (int iz = 0; iz < p->ppc[2]; ++iz) {
+                                const double tx = (ix + 0.5) / p->ppc[0] + i - 1;
+                                const double ty = (iy + 0.5) / p->ppc[1] + j - 1;
+                                const double tz = (iz + 0.5) / p->ppc[2] + k - 1;
+                                const double a1 = std::fabs(tz - bcz);
+                                const double a2 = std::sqrt((tx - bcx) * (tx - bcx) +
+                                                            (ty - bcy) * (ty - bcy));
+                                const double a3 = std::fabs(a2 - r1);
+                                const double a4 = a1 * a1 + a3 * a3;
+                                h[0][n] = tx; h[1][n] = ty; h[2][n] = tz;
+                                h[3][n] = 0.0; h[4][n] = 0.0; h[5][n] = p->gamma;
+                                h[6][n] = qm * p->np * std::exp(-a4 / (r2 * r2) * 0.5);
+                                ++n;
+                            }
+    return beam_from_host(c, p->q, p->m, p->evolution, npmax, h, n, dt, out);
 }
 
 int qp_beam_create_twiss(qp_ctx *c, const qp_beam_twiss_params *p, double dt, uint64_t seed,

@@ -45,6 +45,12 @@ struct qp_species {
     // s and factors fs; empty for a uniform plasma
     std::vector<double> lp_s, lp_fs;
     double den_factor = 1.0;   // the factor at the last load
+    // transverse profile (0 uniform, 10, 11, 12; fdist2d_class.f03:318-950),
+    // lengths in grid units
+    int profile = 0;
+    double pcx = 0, pcy = 0, irx = 0, iry = 0, orx = 0, ory = 0, r0 = 0, sigr = 1;
+    std::vector<double> pr, pfr;
+    std::string profile_err;   // a failed check of the last load
 };
 
 namespace qp {
@@ -1574,6 +1580,47 @@ static int set_q_uniform(qp_species *sp, bool uniform, double v)
     return rc;
 }
 
+// The transverse profile at a lattice point (x, y) in grid units: false
+// drops it; w is its charge weight.  In the reference's operation order:
+//   10 (dist2d_010, fdist2d_class.f03:438-441): dropped inside the inner or
+//      outside the outer ellipse;
+//   11 (dist2d_011, :633-645): w = exp(-(r - r0)^2 * 0.5 / sigr^2);
+//   12 (dist2d_012, :836-856): dropped outside [r(1), r(n)], else w = fr
+//      interpolated linearly at r, with the check that r increases (its
+//      failure is left in sp->profile_err).
+static bool profile_weight(qp_species *sp, double x, double y, double &w)
+{
+    w = 1.0;
+    if (sp->profile == 0) return true;
+    const double cx = sp->pcx, cy = sp->pcy;
+    if (sp->profile == 10) {
+        const double iirx2 = 1.0 / (sp->irx * sp->irx), iiry2 = 1.0 / (sp->iry * sp->iry);
+        const double iorx2 = 1.0 / (sp->orx * sp->orx), iory2 = 1.0 / (sp->ory * sp->ory);
+        const double ddx = (x - cx) * (x - cx), ddy = (y - cy) * (y - cy);
+        return !((ddx * iirx2 + ddy * iiry2 < 1.0) || (ddx * iorx2 + ddy * iory2 > 1.0));
+    }
+    const double rr = std::sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy));
+    if (sp->profile == 11) {
+        const double isigr2 = 0.5 / (sp->sigr * sp->sigr);
+        w = std::exp(-((rr - sp->r0) * (rr - sp->r0)) * isigr2);
+        return true;
+    }
+    const std::vector<double> &r = sp->pr, &fr = sp->pfr;
+    const size_t m = r.size();
+    if (rr < r[0] || rr > r[m - 1]) return false;
+    for (size_t ii = 1; ii < m; ++ii) {
+        if (r[ii] <= r[ii - 1]) {
+            sp->profile_err = "dist2d_012: r is not monotonically increasing!";
+            return false;
+        }
+        if (rr <= r[ii]) {
+            w = fr[ii - 1] + (fr[ii] - fr[ii - 1]) / (r[ii] - r[ii - 1]) * (rr - r[ii - 1]);
+            break;
+        }
+    }
+    return true;
+}
+
 template <typename R>
 static int distribute(qp_species *sp)
 {
@@ -1582,15 +1629,21 @@ static int distribute(qp_species *sp)
     const int64_t np = (int64_t)(nx - 2) * (ny - 1) * xppc * yppc;
     const double qm = sp->den_factor * sp->p.density * sp->p.q / std::fabs(sp->p.q) / (double)xppc /
                       (double)yppc;
-    std::vector<R> hx((size_t)np), hy((size_t)np);
+    std::vector<R> hx((size_t)np), hy((size_t)np), hq;
+    sp->profile_err.clear();
     std::vector<int> hkeys;
     if (TileRel<R>::on) hkeys.resize((size_t)np);
+    const bool weighted = sp->profile == 11 || sp->profile == 12;
+    if (weighted) hq.resize((size_t)np);
     int64_t n = 0;
     for (int i = 2; i <= nx - 1; ++i)
         for (int j = 2; j <= ny; ++j)
             for (int ix = 0; ix < xppc; ++ix)
                 for (int iy = 0; iy < yppc; ++iy) {
                     const double x = (ix + 0.5) / xppc + i - 1, y = (iy + 0.5) / yppc + j - 1;
+                    double w = 1.0;
+                    if (!profile_weight(sp, x, y, w)) continue;
+                    if (weighted) hq[n] = (R)(qm * w);
                     if (TileRel<R>::on) {
                         const int cx = (int)x, cy = (int)y;
                         hkeys[n] = host_cell_key(cx, cy, c->mx1);
@@ -1602,21 +1655,25 @@ static int distribute(qp_species *sp)
                     }
                     ++n;
                 }
-    sp->np = np;
-    const size_t bytes = (size_t)np * sizeof(R);
+    QP_REQUIRE(sp->profile_err.empty(), QP_ERR_ARG, "%s", sp->profile_err.c_str());
+    sp->np = n;
+    if (TileRel<R>::on) hkeys.resize((size_t)n);
+    const size_t bytes = (size_t)n * sizeof(R);
     QP_HIP(hipMemcpyAsync(sp->a[0].p, hx.data(), bytes, hipMemcpyHostToDevice, c->stream));
     QP_HIP(hipMemcpyAsync(sp->a[1].p, hy.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    if (weighted)
+        QP_HIP(hipMemcpyAsync(sp->a[7].p, hq.data(), bytes, hipMemcpyHostToDevice, c->stream));
     QP_HIP(hipStreamSynchronize(c->stream));
     // p6 = p7 = 1 at rest (0 in the FP32 deviation form)
     const double p67 = P67Dev<R>::on ? 0.0 : 1.0;
     const double fillv[6] = {0.0, 0.0, 0.0, p67, p67, qm};
-    for (int k = 2; k < 8; ++k) {
-        int rc = launch_fill(c, sp->a[k].p, (size_t)np, fillv[k - 2]);
+    for (int k = 2; k < (weighted ? 7 : 8); ++k) {
+        int rc = launch_fill(c, sp->a[k].p, (size_t)n, fillv[k - 2]);
         if (rc) return rc;
     }
     int rc = rebin_unsorted<R>(sp, TileRel<R>::on ? &hkeys : nullptr);
     if (rc) return rc;
-    return set_q_uniform(sp, true, qm);
+    return set_q_uniform(sp, !weighted, qm);
 }
 
 // The density factor at propagation distance s of dist2d_000
@@ -1908,6 +1965,47 @@ int qp_species_set_longitudinal(qp_species *sp, int n, const double *s, const do
                "qp_species_set_longitudinal: %d points (0, or at least 2)", n);
     sp->lp_s.assign(s, s + n);
     sp->lp_fs.assign(fs, fs + n);
+    return qp_species_renew(sp, s_now);
+}
+
+// init_fdist2d_010 / _011 / _012 (fdist2d_class.f03:318-388, 525-590,
+// 713-776): lengths to grid units, the centre relative to the box's lower
+// corner; then a reload at s_now
+int qp_species_set_profile(qp_species *sp, const qp_species_profile *p, double s_now)
+{
+    SPCHECK(sp);
+    QP_REQUIRE(p, QP_ERR_ARG, "qp_species_set_profile: null profile");
+    QP_REQUIRE(p->profile == 0 || p->profile == 10 || p->profile == 11 || p->profile == 12,
+               QP_ERR_UNSUPPORTED, "qp_species_set_profile: plasma profile %d (0, 10, 11, 12)",
+               p->profile);
+    const qp_ctx *c = sp->ctx;
+    const double dx = c->dex, dy = c->dey;
+    if (p->profile == 10)
+        QP_REQUIRE(p->inner_radius[0] > 0 && p->inner_radius[1] > 0 && p->outer_radius[0] > 0 &&
+                       p->outer_radius[1] > 0,
+                   QP_ERR_ARG, "profile 10: radii must be positive");
+    if (p->profile == 11)
+        QP_REQUIRE(p->width > 0, QP_ERR_ARG, "profile 11: width must be positive");
+    if (p->profile == 12) {
+        QP_REQUIRE(p->nr >= 2 && p->r && p->fr, QP_ERR_ARG,
+                   "profile 12: at least 2 radial points");
+    }
+    sp->profile = p->profile;
+    sp->pcx = (p->center[0] - c->box[0]) / dx;
+    sp->pcy = (p->center[1] - c->box[2]) / dy;
+    sp->irx = p->inner_radius[0] / dx;
+    sp->iry = p->inner_radius[1] / dy;
+    sp->orx = p->outer_radius[0] / dx;
+    sp->ory = p->outer_radius[1] / dy;
+    sp->r0 = p->radius / dx;
+    sp->sigr = p->width / dx;
+    sp->pr.clear();
+    sp->pfr.clear();
+    if (p->profile == 12)
+        for (int k = 0; k < p->nr; ++k) {
+            sp->pr.push_back(p->r[k] / dx);
+            sp->pfr.push_back(p->fr[k]);
+        }
     return qp_species_renew(sp, s_now);
 }
 

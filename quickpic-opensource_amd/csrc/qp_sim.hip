@@ -5,6 +5,7 @@
 // bxyz (3); cu and epw start in k space.  Every call below enqueues on the
 // context stream; the only host synchronisations are the corrector's stopping
 // test (when a tolerance is given) and the particle check at the slice end.
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -43,6 +44,7 @@ struct qp_sim {
     int hist_slices;        // slices of this step recorded in the history
     int prepared;           // slice whose first part has run (0: none)
     int last_passes;        // corrector passes of the last finished slice
+    int pred_passes = 0;    // passes the next slice's corrector enqueues at once
     bool last_adaptive;     // whether its residuals were read back
     // Captured slices (qp_scan): while `graph` is set, prepare and finish
     // enqueue a slice whose index lives in device memory, so one captured
@@ -374,26 +376,42 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         // and decides; once pass l converges, flag word l is set and those
         // deposits return at once, leaving the fields and particles of the
         // converged pass.  The field solve of pass l+1 is enqueued only when
-        // pass l has not converged.
+        // pass l has not converged.  With QP_TUNE_SPECULATE = 2 the first
+        // wait comes only after P whole passes, P the last slice's count:
+        // the passes after a converged one return at once (the skipped
+        // transforms pass the converged flag on), so the host reads P
+        // residuals, first pass first, after one wait instead of P.
         struct SkipGuard {
             qp_ctx *c;
             ~SkipGuard() { c->skip = nullptr; }
         } guard{c};
         c->skip = flags;
-        RC(pass(1));
-        QP_HIP(hipEventRecord(c->pass_ev[1], c->stream));
-        for (l = 1;; ++l) {
+        const int P = c->tune[QP_TUNE_SPECULATE] == 2
+                          ? std::max(1, std::min(iter_max, s->pred_passes)) : 1;
+        for (int k = 1; k <= P; ++k) RC(pass(k));
+        QP_HIP(hipEventRecord(c->pass_ev[P & 1], c->stream));
+        int lo = 1;   // the first pass whose residual has not been read
+        for (l = P;; ++l) {
             const bool more = l < iter_max;
             if (more) RC(head(l + 1));
             QP_HIP(hipEventSynchronize(c->pass_ev[l & 1]));
-            res = c->h_res[l % RES_SLOTS];
-            if (res < tol || !more) {
-                if (more) prof_discard(c, l + 1);
+            // passes lo..l ran (those after a converged one returned at once
+            // and left their residual slots alone): the first below tol
+            int conv = 0;
+            for (int k = lo; k <= l && !conv; ++k)
+                if (c->h_res[k % RES_SLOTS] < tol) conv = k;
+            if (conv || !more) {
+                const int last = conv ? conv : l;
+                res = c->h_res[last % RES_SLOTS];
+                for (int k = last + 1; k <= l + (more ? 1 : 0); ++k) prof_discard(c, k);
+                l = last;
                 break;
             }
             RC(tail(l + 1));
             QP_HIP(hipEventRecord(c->pass_ev[(l + 1) & 1], c->stream));
+            lo = l + 1;
         }
+        s->pred_passes = l;
     }
     if (l > iter_max) l = iter_max;
     // species charge diagnostic of the last pass (cbq, :692-695): it reads

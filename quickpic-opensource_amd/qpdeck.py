@@ -5,10 +5,13 @@ The deck format is the reference's (input_class.f03:62-145): JSON in which
 character), with the keys read in simulation_class.f03:151-224 (grid, box, time
 step, corrector passes, counts of beams and species) and 392-569 (per beam and
 per species).  The first `nbeams` beams and `nspecies` species are used, as in
-the reference.  This build implements beam profile 0 (bi-Gaussian,
-fdist3d_000) and profile 2 (Twiss, fdist3d_002), and species profile 0
-(fdist2d_000) with a uniform or piecewise longitudinal density; a deck that
-asks for another profile is rejected with the reference's message for it.
+the reference.  This build implements beam profiles 0 (bi-Gaussian,
+fdist3d_000), 2 (Twiss, fdist3d_002), 3 (particles from a file, fdist3d_003)
+and 100 (quiet-start torus, fdist3d_100), and species profiles 0
+(uniform, fdist2d_000), 10 (elliptical annulus), 11 (Gaussian ring) and 12
+(piecewise radial density), each with a uniform or piecewise longitudinal
+density; a deck that asks for another profile is rejected with the
+reference's message for it.
 Diagnostics (`diag` of fields, species and beams) and beam restart files
 (`dump_restart`, `read_restart`) are written and read in the reference's HDF5
 format by qpdiag.py.  The MPI layout (`nodes`) has no effect on one GPU.
@@ -16,6 +19,7 @@ format by qpdiag.py.  The MPI layout (`nodes`) has no effect on one GPU.
 from __future__ import annotations
 
 import json
+import os
 from dataclasses import dataclass, field
 
 import qpslice as qps
@@ -87,12 +91,45 @@ class TwissCfg:
 
 
 @dataclass
+class FileBeamCfg:
+    """Beam profile 3 (init_fdist3d_003, fdist3d_class.f03:823-897)."""
+    file_name: str
+    np_: int
+    center: tuple
+    n0: float
+    q: float = -1.0
+    m: float = 1.0
+    evolution: bool = True
+    npmax: int = 0
+
+
+@dataclass
+class TorusBeamCfg:
+    """Beam profile 100 (init_fdist3d_100, fdist3d_class.f03:976-1082)."""
+    center: tuple
+    range_x: tuple
+    range_y: tuple
+    range_z: tuple
+    ppc: tuple
+    q: float
+    m: float
+    np_: float
+    gamma: float
+    r1: float
+    r2: float
+    evolution: bool = True
+    npmax: int = 0
+
+
+@dataclass
 class SpeciesCfg:
     ppc: tuple
     density: float = 1.0
     q: float = -1.0
     m: float = 1.0
     piecewise: tuple = None     # (s points, density factors) or None
+    profile: int = 0            # 0, 10, 11, 12
+    transverse: dict = field(default_factory=dict)   # the profile's keys (set_profile)
 
 
 @dataclass
@@ -141,8 +178,25 @@ def config(deck: dict) -> DeckCfg:
                 m=float(b.get("m", 1.0)), quiet=bool(b.get("quiet_start", True)),
                 evolution=bool(b.get("evolution", True)), npmax=int(b.get("npmax", 0))))
             continue
+        if prof == 3:     # init_fdist3d_003 (fdist3d_class.f03:823-897)
+            cfg.beams.append(FileBeamCfg(
+                file_name=str(b["file_name"]), np_=int(b["np"]), center=_f3(b["center"]),
+                n0=float(s["n0"]), q=float(b["q"]), m=float(b["m"]),
+                evolution=bool(b.get("evolution", True)), npmax=int(b.get("npmax", 0))))
+            continue
+        if prof == 100:   # init_fdist3d_100 (fdist3d_class.f03:976-1082)
+            cfg.beams.append(TorusBeamCfg(
+                center=_f3(b["center"]), range_x=tuple(float(x) for x in b["range_x"]),
+                range_y=tuple(float(x) for x in b["range_y"]),
+                range_z=tuple(float(x) for x in b["range_z"]),
+                ppc=tuple(int(x) for x in b["ppc"]), q=float(b["q"]), m=float(b["m"]),
+                np_=float(b["np"]), gamma=float(b["gamma"]), r1=float(b["r1"]),
+                r2=float(b["r2"]), evolution=bool(b.get("evolution", True)),
+                npmax=int(b.get("npmax", 0))))
+            continue
         if prof != 0:
-            raise ValueError(f"Invalid beam profile number: {prof} (this build supports 0, 2)")
+            raise ValueError(f"Invalid beam profile number: {prof} "
+                             "(this build supports 0, 2, 3, 100)")
         cfg.beams.append(BeamCfg(
             np_=tuple(int(x) for x in b["np"]), center=_f3(b["center"]), sigma=_f3(b["sigma"]),
             sigma_v=_f3(b["sigma_v"]), gamma=float(b["gamma"]),
@@ -153,8 +207,20 @@ def config(deck: dict) -> DeckCfg:
     for i in range(int(s["nspecies"])):
         p = deck["species"][i]
         prof = int(p["profile"])
-        if prof != 0:
-            raise ValueError(f"Invalid species profile number: {prof} (this build supports 0)")
+        if prof not in (0, 10, 11, 12):
+            raise ValueError(f"Invalid species profile number: {prof} "
+                             "(this build supports 0, 10, 11, 12)")
+        tp = {}
+        if prof in (10, 11, 12):     # keys of init_fdist2d_010/011/012
+            tp["center"] = (float(p["center"][0]), float(p["center"][1]))
+        if prof == 10:
+            tp["inner_radius"] = (float(p["inner_radius"][0]), float(p["inner_radius"][1]))
+            tp["outer_radius"] = (float(p["outer_radius"][0]), float(p["outer_radius"][1]))
+        elif prof == 11:
+            tp["radius"], tp["width"] = float(p["radius"]), float(p["width"])
+        elif prof == 12:
+            tp["radial"] = (tuple(float(x) for x in p["piecewise_r"]),
+                            tuple(float(x) for x in p["piecewise_radial_density"]))
         lp = p.get("longitudinal_profile", "uniform")
         pw = None
         if lp == "piecewise":     # fdist2d_class.f03:197-201
@@ -165,7 +231,8 @@ def config(deck: dict) -> DeckCfg:
                              "(uniform, piecewise)")
         cfg.species.append(SpeciesCfg(ppc=tuple(int(x) for x in p["ppc"]),
                                       density=float(p["density"]), q=float(p["q"]),
-                                      m=float(p["m"]), piecewise=pw))
+                                      m=float(p["m"]), piecewise=pw, profile=prof,
+                                      transverse=tp))
     return cfg
 
 
@@ -188,8 +255,23 @@ class Run:
         for sp, s in zip(self.species, cfg.species):
             if s.piecewise is not None:
                 sp.set_longitudinal(s.piecewise[0], s.piecewise[1], s0)
+            if s.profile != 0:
+                sp.set_profile(s.profile, s_now=s0, **s.transverse)
         self.beams = []
         for i, b in enumerate(cfg.beams):
+            if isinstance(b, FileBeamCfg):
+                path = b.file_name if os.path.isabs(b.file_name) else os.path.join(root,
+                                                                                    b.file_name)
+                self.beams.append(qps.FileBeam(self.ctx, path, b.np_, center=b.center, n0=b.n0,
+                                               q=b.q, m=b.m, evolution=b.evolution, dt=cfg.dt,
+                                               npmax=b.npmax))
+                continue
+            if isinstance(b, TorusBeamCfg):
+                self.beams.append(qps.TorusBeam(
+                    self.ctx, center=b.center, range_x=b.range_x, range_y=b.range_y,
+                    range_z=b.range_z, ppc=b.ppc, q=b.q, m=b.m, np_=b.np_, gamma=b.gamma,
+                    r1=b.r1, r2=b.r2, evolution=b.evolution, dt=cfg.dt, npmax=b.npmax))
+                continue
             if isinstance(b, TwissCfg):
                 self.beams.append(qps.TwissBeam(
                     self.ctx, center=b.center, alpha=b.alpha, beta=b.beta, emittance=b.emittance,

@@ -74,10 +74,26 @@ class Instance:
         self.ctx.close()
 
 
+def draw_one(i, grids, ppcs, indz, beam_np, seed=0, attempt=0, precision=qps.FP32, iter_=2,
+             tol=0.0, seed_mode=0):
+    """A replacement for instance i of a scan (attempt >= 1): grid 2^g (g
+    from grids), ppc^2 particles per cell (ppc from ppcs), driver drawn from
+    the ranges above, with a stream and beam seed of its own."""
+    rng = np.random.default_rng([seed, i, attempt])
+    g = int(rng.choice(grids))
+    p = int(rng.choice(ppcs))
+    peak, sr, sz = (float(rng.uniform(1, 20)), float(rng.uniform(0.2, 1.0)),
+                    float(rng.uniform(0.3, 1.5)))
+    return Instance(g, indz, p, peak, sr, sz, seed=1 + 7919 * seed + 104729 * attempt + i,
+                    beam_np=beam_np, precision=precision, iter_=iter_, tol=tol,
+                    seed_mode=seed_mode)
+
+
 def draw(n, grids, ppcs, indz, beam_np, seed=0, precision=qps.FP32, iter_=2, tol=0.0,
          seed_mode=0):
     """n instances with grids 2^g (g from grids), ppc^2 particles per cell
-    (ppc from ppcs) and drivers drawn from the ranges above."""
+    (ppc from ppcs) and drivers drawn from the ranges above, from one stream
+    seeded by `seed` (draw_one makes the replacements)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -89,6 +105,36 @@ def draw(n, grids, ppcs, indz, beam_np, seed=0, precision=qps.FP32, iter_=2, tol
                             beam_np=beam_np, precision=precision, iter_=iter_, tol=tol,
                             seed_mode=seed_mode))
     return out
+
+
+def screen(insts, nslices, redraw, log=None, rounds=8):
+    """Runs every instance once over nslices slices with the strict particle
+    move check (PPPCHECK2L: a particle crossing more than one tile in a
+    slice stops the reference's run); an instance that stops is replaced by
+    redraw(i, attempt), a fresh draw, until all pass.  Returns the count of
+    replaced draws."""
+    replaced = 0
+    for attempt in range(rounds):
+        bad = []
+        for i, x in enumerate(insts):
+            x.ctx.tune(qps.TUNE_STRICT, 1)
+            try:
+                x.begin()
+                for j in range(1, nslices + 1):
+                    x.slice(j)
+            except qps.QPError as e:
+                bad.append(i)
+                if log:
+                    log(f"[scan] instance {i} {x.params} stops: {e}")
+            finally:
+                x.ctx.tune(qps.TUNE_STRICT, -1)
+        if not bad:
+            return replaced
+        for i in bad:
+            insts[i].close()
+            insts[i] = redraw(i, attempt + 1)
+            replaced += 1
+    raise RuntimeError(f"scan: draws still stop after {rounds} rounds")
 
 
 def autotune(inst, j0, reps=2):
@@ -195,6 +241,12 @@ def main(argv=None):
                  precision=qps.FP64 if args.fp64 else qps.FP32, iter_=iter_, tol=args.tol,
                  seed_mode=args.seed_mode)
     log(f"[scan] {args.n} instances set up in {time.time() - t0:.1f}s")
+    replaced = screen(insts, args.slices,
+                      lambda i, a: draw_one(i, args.grids, args.ppcs, args.indz,
+                                            tuple(args.beam_np), args.seed, a,
+                                            qps.FP64 if args.fp64 else qps.FP32, iter_, args.tol,
+                                            args.seed_mode), log=log)
+    log(f"[scan] screened: {replaced} draws replaced ({time.time() - t0:.1f}s)")
     if args.tune:
         for x in insts:
             x.begin()
@@ -215,7 +267,8 @@ def main(argv=None):
            "graph_over_sequential": ts / tg, "graph_passes_equal_sequential": same,
            "passes_mean": float(np.mean(seq_passes)),
            "grids": sorted({x.params["grid"] for x in insts}),
-           "ppc": sorted({x.params["ppc"] for x in insts})}
+           "ppc": sorted({x.params["ppc"] for x in insts}),
+           "draws_replaced_by_move_check": replaced}
     if args.batched and args.tol <= 0:
         tb = run_batched(insts, args.slices)
         res["batched_streams_slices_per_s"] = total / tb

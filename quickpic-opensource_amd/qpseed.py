@@ -227,6 +227,7 @@ class Trainer:
         self.n, self.warm, self.per_pair, self.lr = 0, warm, per_pair, lr
         self.rng = np.random.default_rng(seed)
         self.losses = []
+        self.target_ms = []   # mean square of the target of each step's pair
 
     def add(self, x, y):
         b = len(self.x)
@@ -244,7 +245,16 @@ class Trainer:
         H, W = self.H, self.W
         self.net.forward(self.x[i], H * W, W, H, W)
         self.losses.append(self.net.backward(self.y[i].reshape(2, -1)))
+        self.target_ms.append(float((self.y[i] * self.y[i]).mean()))
         self.net.adam(lr=self.lr)
+
+    def normalised_loss(self, last=200):
+        """Mean loss over the last steps divided by their targets' mean
+        square: 1 is what predicting zero (the plain extrapolation) scores."""
+        n = min(last, len(self.losses))
+        if n == 0:
+            return float("nan")
+        return float(np.sum(self.losses[-n:]) / max(np.sum(self.target_ms[-n:]), 1e-300))
 
 
 def save(net, path):

@@ -58,11 +58,28 @@ class SpeciesParams(C.Structure):
     _fields_ = [("ppc", _I * 2), ("density", _D), ("q", _D), ("m", _D)]
 
 
+class SpeciesProfile(C.Structure):
+    _fields_ = [("profile", _I), ("center", _D * 2), ("inner_radius", _D * 2),
+                ("outer_radius", _D * 2), ("radius", _D), ("width", _D), ("nr", _I),
+                ("r", C.POINTER(_D)), ("fr", C.POINTER(_D))]
+
+
 class BeamParams(C.Structure):
     _fields_ = [("center", _D * 3), ("sigma", _D * 3), ("sigma_v", _D * 3),
                 ("centroid_x", _D * 3), ("centroid_y", _D * 3),
                 ("gamma", _D), ("peak", _D), ("q", _D), ("m", _D),
                 ("np", _I * 3), ("quiet", _I), ("evolution", _I), ("npmax", _I64)]
+
+
+class BeamFileParams(C.Structure):
+    _fields_ = [("file_name", C.c_char_p), ("center", _D * 3), ("n0", _D), ("q", _D),
+                ("m", _D), ("np", _I), ("evolution", _I), ("npmax", _I64)]
+
+
+class BeamTorusParams(C.Structure):
+    _fields_ = [("center", _D * 3), ("range_x", _D * 2), ("range_y", _D * 2),
+                ("range_z", _D * 2), ("ppc", _I * 3), ("q", _D), ("m", _D), ("np", _D),
+                ("gamma", _D), ("r1", _D), ("r2", _D), ("evolution", _I), ("npmax", _I64)]
 
 
 class BeamTwissParams(C.Structure):
@@ -356,6 +373,31 @@ class Species:
     def renew(self, s=0.0):
         _call(lib().qp_species_renew, self.h, _D(s))
 
+    def set_profile(self, profile, center=(0.0, 0.0), inner_radius=(0.0, 0.0),
+                    outer_radius=(0.0, 0.0), radius=0.0, width=1.0, radial=None, s_now=0.0):
+        """Transverse profile 10 (elliptical annulus), 11 (Gaussian ring) or
+        12 (radial = (r points, densities)), fdist2d_class.f03:318-950;
+        reloads the plasma at s_now.  0: uniform."""
+        p = SpeciesProfile()
+        p.profile = int(profile)
+        p.center[0], p.center[1] = center
+        p.inner_radius[0], p.inner_radius[1] = inner_radius
+        p.outer_radius[0], p.outer_radius[1] = outer_radius
+        p.radius, p.width = radius, width
+        keep = None
+        if radial is not None:
+            r = np.ascontiguousarray(radial[0], dtype=np.float64)
+            fr = np.ascontiguousarray(radial[1], dtype=np.float64)
+            if r.shape != fr.shape:
+                raise QPError(-1, "dist2d_012: The piecewise_radial_density and r array have "
+                                  "different sizes!")
+            keep = (r, fr)
+            p.nr = len(r)
+            p.r = r.ctypes.data_as(C.POINTER(_D))
+            p.fr = fr.ctypes.data_as(C.POINTER(_D))
+        _call(lib().qp_species_set_profile, self.h, C.byref(p), _D(s_now))
+        del keep
+
     def set_longitudinal(self, s_pts, fs, s_now=0.0):
         """Piecewise longitudinal density (fdist2d_class.f03:190-201,
         233-251); reloads the plasma at s_now.  Empty lists: uniform."""
@@ -512,6 +554,46 @@ class TwissBeam(Beam):
         self.h = _P()
         _call(lib().qp_beam_create_twiss, ctx.h, C.byref(p), _D(dt), C.c_uint64(seed),
               C.byref(self.h))
+        ctx._adopt(self)
+
+
+class FileBeam(Beam):
+    """Beam profile 3 (fdist3d_003): np particles read from a text file of
+    records "z x y vz vx vy q" in units of 5.32150254e9/sqrt(n0)."""
+
+    def __init__(self, ctx: Ctx, file_name, np_, center=(0.0, 0.0, 0.0), n0=1e16, q=-1.0,
+                 m=1.0, evolution=True, dt=10.0, npmax=0):
+        self.ctx = ctx
+        p = BeamFileParams()
+        p.file_name = os.fsencode(file_name)
+        for i in range(3):
+            p.center[i] = center[i]
+        p.n0, p.q, p.m, p.np = n0, q, m, int(np_)
+        p.evolution = 1 if evolution else 0
+        p.npmax = npmax
+        self.h = _P()
+        _call(lib().qp_beam_create_file, ctx.h, C.byref(p), _D(dt), C.byref(self.h))
+        ctx._adopt(self)
+
+
+class TorusBeam(Beam):
+    """Beam profile 100 (fdist3d_100): a quiet-start Gaussian torus (radius
+    r1, width r2) over the range box.  Lengths in c/wp."""
+
+    def __init__(self, ctx: Ctx, center=(0.0, 0.0, 2.0), range_x=(-3.0, 3.0),
+                 range_y=(-3.0, 3.0), range_z=(1.0, 3.0), ppc=(2, 2, 2), q=-1.0, m=1.0,
+                 np_=1.0, gamma=2.0e4, r1=1.0, r2=0.3, evolution=True, dt=10.0, npmax=0):
+        self.ctx = ctx
+        p = BeamTorusParams()
+        for i in range(3):
+            p.center[i], p.ppc[i] = center[i], ppc[i]
+        for i in range(2):
+            p.range_x[i], p.range_y[i], p.range_z[i] = range_x[i], range_y[i], range_z[i]
+        p.q, p.m, p.np, p.gamma, p.r1, p.r2 = q, m, np_, gamma, r1, r2
+        p.evolution = 1 if evolution else 0
+        p.npmax = npmax
+        self.h = _P()
+        _call(lib().qp_beam_create_torus, ctx.h, C.byref(p), _D(dt), C.byref(self.h))
         ctx._adopt(self)
 
 

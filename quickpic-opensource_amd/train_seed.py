@@ -74,20 +74,35 @@ def train(args, log=print):
                         lr=args.lr, seed=args.seed)
     rng = np.random.default_rng(args.seed)
     t0 = time.time()
+    stopped = 0
     for r in range(args.runs):
         peak, sr, sz = draw_beam(rng)
-        run.new_beam(peak, sr, sz, seed=1000 + r)
+        try:
+            run.new_beam(peak, sr, sz, seed=1000 + r)
+        except qps.QPError:
+            # the move check of a run that stopped is read at the next
+            # reload; the reload after it is clean
+            run.new_beam(peak, sr, sz, seed=1000 + r)
         j0, j1 = run.slices(sz, args.span)
-        # slices ahead of the sampled range only advance the plasma
-        for j in range(1, j0):
-            run.sim.slice(j, iter_max=args.iter_max, tol=args.tol)
-        passes = qpseed.collect(ctx, run.sim, j0, j1, f=args.f, iter_max=args.iter_max,
-                                tol=args.tol, every=args.every, sink=tr.add)
+        try:
+            # slices ahead of the sampled range only advance the plasma
+            for j in range(1, j0):
+                run.sim.slice(j, iter_max=args.iter_max, tol=args.tol)
+            passes = qpseed.collect(ctx, run.sim, j0, j1, f=args.f, iter_max=args.iter_max,
+                                    tol=args.tol, every=args.every, sink=tr.add)
+        except qps.QPError as e:
+            # a plasma electron crossed more than one tile in a slice: the
+            # reference stops such a run (PPPCHECK2L); its slices up to there
+            # have trained the network
+            stopped += 1
+            log(f"[train] run {r}: nb {peak:.1f} sr {sr:.2f} sz {sz:.2f} stops: {e}")
+            continue
         if r % max(1, args.runs // 20) == 0 or r == args.runs - 1:
             loss = np.mean(tr.losses[-50:]) if tr.losses else float("nan")
             log(f"[train] run {r}: nb {peak:.1f} sr {sr:.2f} sz {sz:.2f}  passes "
                 f"{np.mean(passes):.2f}  pairs {tr.n}  loss {loss:.4g}  {time.time() - t0:.0f}s")
     ctx.sync()
+    log(f"[train] {args.runs} runs, {stopped} stopped by the move check, {tr.n} pairs")
     return net, tr
 
 
@@ -141,16 +156,27 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--eval", type=float, nargs=3, default=None,
                     metavar=("PEAK", "SIGMA_R", "SIGMA_Z"), help="held-out beam to evaluate")
+    ap.add_argument("--eval-ind", type=int, nargs="*", default=None,
+                    help="log2 grids to evaluate on (default: the training grid)")
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
-    net, _ = train(args, log=log)
+    net, tr = train(args, log=log)
+    res = {"runs": args.runs, "pairs": tr.n, "grid": 2 ** args.ind,
+           "normalised_loss_last_200": tr.normalised_loss(200)}
+    log(f"[train] normalised loss over the last 200 steps: {res['normalised_loss_last_200']:.4g}")
     if args.out:
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         qpseed.save(net, args.out)
         log(f"[train] weights -> {args.out}")
     if args.eval:
-        evaluate(net, args.ind, args.indz, args.ppc, args.beam_np, args.f, *args.eval,
-                 seed=424242, iter_max=args.iter_max, tol=args.tol, span=args.span, log=log)
+        res["eval"] = {}
+        for ind in (args.eval_ind or [args.ind]):
+            res["eval"][2 ** ind] = evaluate(net, ind, ind, args.ppc, args.beam_np, args.f,
+                                             *args.eval, seed=424242, iter_max=args.iter_max,
+                                             tol=args.tol, span=args.span * 2 ** (ind - args.ind),
+                                             log=log)
+    import json
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

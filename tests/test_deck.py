@@ -29,12 +29,52 @@ def test_deck_config():
 
 def test_unsupported_profile_is_rejected():
     deck = qpdeck.load(DECK)
-    deck["simulation"]["nbeams"] = 2          # the second beam has profile 3
-    with pytest.raises(ValueError, match="Invalid beam profile number: 3"):
+    deck["simulation"]["nbeams"] = 2
+    deck["beam"][1]["profile"] = 1            # RANDOM_NUMBER's stream: not built
+    with pytest.raises(ValueError, match="Invalid beam profile number: 1"):
         qpdeck.config(deck)
     deck = qpdeck.load(DECK)
     deck["species"][0]["longitudinal_profile"] = "gaussian"
     with pytest.raises(ValueError, match="gaussian"):
+        qpdeck.config(deck)
+
+
+def test_file_and_torus_beams_are_read():
+    """init_fdist3d_003 and init_fdist3d_100 keys (fdist3d_class.f03:
+    823-1082); profile 3 takes n0 from the simulation block."""
+    deck = qpdeck.load(DECK)
+    deck["simulation"]["nbeams"] = 2
+    deck["beam"][1].update({"file_name": "beam.txt", "center": [0.0, 0.0, 1.0],
+                            "evolution": False})
+    cfg = qpdeck.config(deck)
+    b = cfg.beams[1]
+    assert isinstance(b, qpdeck.FileBeamCfg)
+    assert (b.file_name, b.np_, b.n0, b.evolution) == ("beam.txt", 1000, 1e16, False)
+    deck["beam"][1] = {"profile": 100, "center": [0.0, 0.0, 2.0], "range_x": [-3, 3],
+                       "range_y": [-2, 2], "range_z": [1, 3], "ppc": [2, 2, 1], "q": -1.0,
+                       "m": 1.0, "np": 0.5, "gamma": 100.0, "r1": 1.0, "r2": 0.3}
+    b = qpdeck.config(deck).beams[1]
+    assert isinstance(b, qpdeck.TorusBeamCfg)
+    assert b.ppc == (2, 2, 1) and b.range_y == (-2.0, 2.0) and b.r2 == 0.3
+
+
+def test_transverse_plasma_profiles_are_read():
+    """init_fdist2d_010/011/012 keys (fdist2d_class.f03:318-776)."""
+    deck = qpdeck.load(DECK)
+    sp = deck["species"][0]
+    sp.update({"profile": 11, "center": [0.5, -0.5], "radius": 3.0, "width": 0.7})
+    cfg = qpdeck.config(deck)
+    assert cfg.species[0].profile == 11
+    assert cfg.species[0].transverse == {"center": (0.5, -0.5), "radius": 3.0, "width": 0.7}
+    sp.update({"profile": 12, "piecewise_r": [0.0, 2.0, 5.0],
+               "piecewise_radial_density": [1.0, 0.5, 0.0]})
+    cfg = qpdeck.config(deck)
+    assert cfg.species[0].transverse["radial"] == ((0.0, 2.0, 5.0), (1.0, 0.5, 0.0))
+    sp.update({"profile": 10, "inner_radius": [1.0, 2.0], "outer_radius": [3.0, 4.0]})
+    cfg = qpdeck.config(deck)
+    assert cfg.species[0].transverse["outer_radius"] == (3.0, 4.0)
+    sp["profile"] = 13
+    with pytest.raises(ValueError, match="Invalid species profile number: 13"):
         qpdeck.config(deck)
 
 

@@ -42,7 +42,9 @@ def make_pair(precision, peak=4.0, sigma=(0.8, 0.8, 0.5), np_=(32, 32, 64), iter
     return ctx, osim, gsim, sp
 
 
-def compare_state(osim, gsim, sp, j, bar):
+def compare_state(osim, gsim, sp, j, bar, bz_bar=None):
+    """Fields of slice j against the oracle, each relative to its own peak,
+    at `bar`; the stored Bz plane at bz_bar (default: bar)."""
     F = gsim.field
     checks = {
         "psi": (F(qps.F_PSI).download(0), osim.psi),
@@ -58,7 +60,8 @@ def compare_state(osim, gsim, sp, j, bar):
         errs[f"bexyz{comp}"] = rel(e3.plane(comp + 1, j + 1), osim.e3[comp, j])
         errs[f"bbxyz{comp}"] = rel(gsim.bbxyz.plane(comp + 1, j + 1), osim.b3[comp, j])
     print(f"slice {j}: " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
-    bad = {k: v for k, v in errs.items() if not v < bar}
+    bars = {k: (bz_bar if (k == "bbxyz2" and bz_bar) else bar) for k in errs}
+    bad = {k: v for k, v in errs.items() if not v < bars[k]}
     assert not bad, (j, bad)
 
 
@@ -95,7 +98,7 @@ def test_slices_fp32_vs_fp64_oracle():
         osim.slice(j)
         gsim.slice(j)
         if j == 4:
-            compare_state(osim, gsim, sp, 4, 1e-5)
+            compare_state(osim, gsim, sp, 4, 1e-5, bz_bar=1e-4)
         e3, b3 = gsim.bexyz, gsim.bbxyz
         got[j] = {"Ez": e3.plane(3, j + 1), "Ex-By": e3.plane(1, j + 1) - b3.plane(2, j + 1),
                   "Bz": b3.plane(3, j + 1), "psi": gsim.field(qps.F_PSI).download(0)[0]}
@@ -112,7 +115,8 @@ def test_slices_fp32_vs_fp64_oracle():
 @pytest.mark.parametrize("knob", ["TUNE_AMJDP", "TUNE_DEPOSIT_Q"])
 def test_every_fp32_deposit_shape_matches_oracle(knob):
     """The FP32 launch shapes (their own table) against the FP64 oracle at
-    the FP32 bar, 1e-5 relative after 4 slices."""
+    the FP32 bar, 1e-5 relative after 4 slices (the stored Bz 1e-4, as
+    above)."""
     banner(f"fp32 deposit shapes {knob}")
     ctx0 = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP32)
     _, nv = ctx0.tune_get(getattr(qps, knob))
@@ -129,7 +133,7 @@ def test_every_fp32_deposit_shape_matches_oracle(knob):
         gsim.begin_step()
         for j in range(1, 5):
             gsim.slice(j)
-        compare_state(osim, gsim, sp, 4, 1e-5)
+        compare_state(osim, gsim, sp, 4, 1e-5, bz_bar=1e-4)
         ctx.close()
 
 
@@ -174,11 +178,11 @@ def _fp32_run(nsl, precision=None, **tune):
                   peak=10.0, dt=10.0)
     gsim = qps.Sim(ctx, [sp], [bm], iter_=2)
     gsim.begin_step()
-    for j in range(1, nsl + 1):
-        gsim.slice(j, iter_max=20, tol=1e-6)
+    passes = [gsim.slice(j, iter_max=20, tol=1e-6)[0] for j in range(1, nsl + 1)]
     out = [gsim.field(w).download(0) for w in (qps.F_PSIT, qps.F_FXY, qps.F_BXYZ)]
     out.append(gsim.field(qps.F_EPW).download(1))
     out.append(sp.download())
+    out.append(np.array(passes))
     ctx.close()
     return out
 
@@ -210,16 +214,23 @@ def test_push_resort_paths_identical(precision):
         assert x.shape == y.shape and np.array_equal(x, y)
 
 
-def test_speculative_deposits_change_nothing():
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_speculative_deposits_change_nothing(precision):
     """With QP_TUNE_SPECULATE = 1 the deposits of pass l+1 are enqueued
     before the residual of pass l is read and return at once when pass l
-    has converged: pass counts, fields and particles are those of waiting
-    for each residual, bit for bit."""
-    banner("speculative deposits")
-    a = _fp32_run(12, TUNE_SPECULATE=0)
-    b = _fp32_run(12, TUNE_SPECULATE=1)
-    for x, y in zip(a, b):
-        assert np.array_equal(x, y)
+    has converged; with 2 the last slice's number of whole passes is
+    enqueued before the first read, the passes after a converged one
+    returning at once: pass counts, fields and particles are those of
+    waiting for each residual, bit for bit, in both precisions."""
+    banner(f"speculative deposits {precision}")
+    prec = getattr(qps, precision)
+    a = _fp32_run(24, prec, TUNE_SPECULATE=0)
+    print("passes per slice:", list(a[-1]), flush=True)
+    assert len(set(a[-1].tolist())) > 1   # counts vary, so predictions miss both ways
+    for v in (1, 2):
+        b = _fp32_run(24, prec, TUNE_SPECULATE=v)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), v
 
 
 @pytest.mark.parametrize("precision", ["FP32", "FP64"])
