@@ -123,6 +123,7 @@ struct qp_ctx {
     // kernel-class timing (qp_ctx_profile)
     unsigned prof_on;     // bit k: launches of class k are timed
     int prof_tag;         // tag of the launches being enqueued (prof_discard)
+    int prof_epoch;       // slices run so far: tags are unique per slice
     std::vector<qp::ProfEv> prof_pending;
     std::vector<hipEvent_t> prof_pool;
     double prof_ms[QP_PROF_COUNT];

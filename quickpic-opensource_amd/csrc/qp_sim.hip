@@ -313,9 +313,14 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
     psit->state = bxyz->state = 0;
     // corrector passes (:676-710)
     int *flags = c->flags.as<int>();
+    // launches of pass l carry tag tag0 + l, unique to this slice, so that
+    // the timing of a speculative pass that returned at once is dropped
+    // without touching the earlier slices' pending timings
+    c->prof_epoch = (c->prof_epoch + 1) & 0xffffff;
+    const int tag0 = c->prof_epoch << 6;
     // a pass is its deposits (head) and its field solve (tail)
     auto head = [&](int l) -> int {
-        c->prof_tag = l;
+        c->prof_tag = tag0 + l;
         if (c->skip) c->skip = flags + (l - 1);
         if (nsp == 0) {
             RC(qp_field2d_as(cu, 0.0));
@@ -331,7 +336,7 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         return QP_OK;
     };
     auto tail = [&](int l) -> int {
-        c->prof_tag = l;
+        c->prof_tag = tag0 + l;
         if (c->skip) c->skip = flags + (l - 1);
         RC(fft_r2k_batch(c, fwd, fwd_kinds, 3));
         cu->state = dcu->state = amu->state = 1;
@@ -403,7 +408,7 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
             if (conv || !more) {
                 const int last = conv ? conv : l;
                 res = c->h_res[last % RES_SLOTS];
-                for (int k = last + 1; k <= l + (more ? 1 : 0); ++k) prof_discard(c, k);
+                for (int k = last + 1; k <= l + (more ? 1 : 0); ++k) prof_discard(c, tag0 + k);
                 l = last;
                 break;
             }

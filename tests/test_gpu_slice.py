@@ -181,6 +181,8 @@ def _fp32_run(nsl, precision=None, **tune):
     passes = [gsim.slice(j, iter_max=20, tol=1e-6)[0] for j in range(1, nsl + 1)]
     out = [gsim.field(w).download(0) for w in (qps.F_PSIT, qps.F_FXY, qps.F_BXYZ)]
     out.append(gsim.field(qps.F_EPW).download(1))
+    # the deposits' real-space planes, border nodes included
+    out += [gsim.field(w).download(0) for w in (qps.F_QE, qps.F_CU, qps.F_DCU, qps.F_AMU)]
     out.append(sp.download())
     out.append(np.array(passes))
     ctx.close()
@@ -231,6 +233,31 @@ def test_speculative_deposits_change_nothing(precision):
         b = _fp32_run(24, prec, TUNE_SPECULATE=v)
         for x, y in zip(a, b):
             assert np.array_equal(x, y), v
+
+
+@pytest.mark.parametrize("speculate", [0, 1, 2])
+def test_profiler_counts_one_deposit_per_pass(speculate):
+    """Kernel-class timing of a run to a tolerance counts exactly one
+    current deposit per corrector pass of every slice: the timings of
+    speculative deposits that returned at once (their pass was not needed)
+    are dropped, and no other slice's."""
+    banner(f"profiler counts, speculate {speculate}")
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP32)
+    try:
+        ctx.tune(qps.TUNE_SPECULATE, speculate)
+        sp = qps.Species(ctx, ppc=(2, 2))
+        bm = qps.Beam(ctx, center=(0.0, 0.0, 1.5), sigma=(0.5, 0.5, 0.5), np_=(32, 32, 64),
+                      peak=10.0, dt=10.0)
+        gsim = qps.Sim(ctx, [sp], [bm], iter_=2)
+        gsim.begin_step()
+        ctx.profile(True)
+        passes = [gsim.slice(j, iter_max=20, tol=1e-6)[0] for j in range(1, 25)]
+        _, n, _ = ctx.profile_get(qps.PROF_NAMES.index("amjdp"))
+        ctx.profile(False)
+        print("passes", passes, "deposits counted", n, flush=True)
+        assert n == sum(passes)
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("precision", ["FP32", "FP64"])
