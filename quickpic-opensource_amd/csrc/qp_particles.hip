@@ -546,6 +546,12 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
 // chunk's cells over all four waves at half their lanes, 2 %; each lane
 // storing its particles' new p6 itself instead of the copy-back, 1.6 %.)
 // UQ: every particle's charge is quni (not staged).
+// RG = 0: the rows per group are chosen per tile, as 16 / T for the smallest
+// T (lanes per cell: 1, 2, 4, 8 or 16) with which the tile's particles,
+// spread evenly over its T groups, fit one staging chunk per group; a chunk
+// then mostly holds every cell of its group and all lanes compute (with a
+// fixed RG the chunks of a dense group hold only some of its cells, and the
+// lanes of the others wait).
 template <typename R, int RG, int CH, int WPE, bool PSI, bool UQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
@@ -554,20 +560,26 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
 {
     const R *psi = PSI ? psi_arg : nullptr;
     if (skip && *skip) return;
-    constexpr int T = TILE / RG, NP = Stage<CH>::N;
+    constexpr int NP = Stage<CH>::N;
     extern __shared__ unsigned char smem_raw[];
     R *sx = reinterpret_cast<R *>(smem_raw), *sy = sx + NP, *spx = sy + NP, *spy = spx + NP;
     R *sp6 = spy + NP, *sp7 = sp6 + NP, *sq = sp7 + NP, *nacc = sq + NP;   // nacc: 8 x 289
     const int tid = threadIdx.x, tile = blockIdx.x;
     const int tx = tile % g.mx1, ty = tile / g.mx1, kb = tile * TCELLS;
     for (int n = tid; n < 8 * 289; n += TCELLS) nacc[n] = R(0);
+    int T = RG > 0 ? TILE / (RG > 0 ? RG : 1) : 1;
+    if (RG == 0) {
+        const int ntile = off[kb + TCELLS] - off[kb];
+        while (T < TILE && ntile > CH * T) T *= 2;
+    }
+    const int rg = TILE / T;   // rows per group
     const int cl = tid / T, sub = tid % T;
     const R qtmh = R(0.5) * qbm * dt, dti = R(1) / dt, idex = R(1) / dex;
-    for (int row = 0; row < TILE; row += RG) {
+    for (int row = 0; row < TILE; row += rg) {
         const int lrow = row + cl / TILE, lcol = cl % TILE;
         const int key = kb + lrow * TILE + lcol;
         const int i0 = off[key], i1 = off[key + 1];
-        const int G0 = off[kb + row * TILE], G1 = off[kb + (row + RG) * TILE];
+        const int G0 = off[kb + row * TILE], G1 = off[kb + (row + rg) * TILE];
         const int cx = tx * TILE + lcol, cy = ty * TILE + lrow;
         Fields6<R> F = load_fields6(fxy, bxy, psit, cs, g.ldr, cx, cy);
         prep_fields(F);
@@ -609,7 +621,6 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
             if (psi)
                 for (int p = tid; p < n; p += TCELLS) s.p7[c0 + p] = sp7[lpad(p)];
         }
-#pragma unroll
         for (int o = T / 2; o > 0; o >>= 1)
 #pragma unroll
             for (int c = 0; c < 8; ++c)
@@ -1483,9 +1494,14 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold,
 }
 
 // species charge into a real-space plane (overwritten), guard-folded
-// Deposit launch shapes: rows per group RG (16/RG lanes per cell), staging
-// chunk CH and a waves-per-SIMD floor WPE (4 caps the kernel at 128 VGPRs).
-// Variant 0 of each table is the default; qp_ctx_tune selects another.
+// Deposit launch shapes: rows per group RG (16/RG lanes per cell; 0: chosen
+// per tile), staging chunk CH and a waves-per-SIMD floor WPE (4 caps the
+// kernel at 128 VGPRs).  Variant 0 of each table is the default; qp_ctx_tune
+// selects another.  The FP32 current deposit's default chooses its rows per
+// group per tile: 451.7 against 434.8-437.2 slices/s on whole 1024^2, 16 ppc
+// steps with RG = 8 (variant 11), the deposit 661 against 720-726 ms per
+// step; in FP64 the per-tile choice was slower (1679 against 1605 ms), so
+// RG = 8 stays its default.
 struct DepShape {
     int rg, ch, wpe;
     int wave;          // current deposit: 1 k_amjdp_w (independent waves, staging
@@ -1497,10 +1513,10 @@ template <typename R> struct DepTab;
 template <> struct DepTab<float> {
     static constexpr DepShape q[] = {{16, 4096, 1, 0}, {16, 1024, 1, 0}, {8, 2048, 1, 0},
                                      {16, 2048, 1, 0}};
-    static constexpr DepShape j[] = {{8, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
+    static constexpr DepShape j[] = {{0, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
-                                     {8, 512, 1, 2},   {8, 512, 1, 3}};
+                                     {8, 512, 1, 2},   {8, 512, 1, 3},  {8, 1024, 4, 0}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
@@ -1508,7 +1524,7 @@ template <> struct DepTab<double> {
     static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
                                      {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
-                                     {8, 256, 1, 2},   {8, 256, 1, 0}};
+                                     {8, 256, 1, 2},   {8, 256, 1, 0},  {0, 1024, 1, 0}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1819,6 +1835,7 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 8: rc = amjdp_v<R, 8>(sp, ef, bf, psit, dex, out, psi); break;
     case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
     case 10: rc = amjdp_v<R, 10>(sp, ef, bf, psit, dex, out, psi); break;
+    case 11: rc = amjdp_v<R, 11>(sp, ef, bf, psit, dex, out, psi); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;
