@@ -66,7 +66,9 @@ void *noalloc_scratch(int device);
         if (!(cond)) { qp::set_error(__VA_ARGS__); return (code); }         \
     } while (0)
 
-template <typename T> struct cplx { T x, y; };
+// aligned to its size, so that a pair moves with one 8- or 16-byte access
+// (LDS ds_read_b64 / ds_write_b64 with immediate offsets in the transforms)
+template <typename T> struct alignas(2 * sizeof(T)) cplx { T x, y; };
 
 // device buffer owned by the library
 struct DevBuf {

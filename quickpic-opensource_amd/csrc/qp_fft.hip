@@ -113,7 +113,7 @@ k_fft_rows(CompSet<R> cs, int nrows, int ld_src, int ld_dst, int types, R scale,
     }
     if (blockIdx.x == 0 && blockIdx.y == 0) run_res_job(rj);
     constexpr int M = S::M, n = S::N, TF = S::TF;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int comp = blockIdx.y;
     const int type = (types >> comp) & 1;   // 0 sine, 1 cosine
     const int f = threadIdx.x / TF, t = threadIdx.x % TF;
