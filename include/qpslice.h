@@ -104,9 +104,10 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
 /* Launch-shape tuning, e.g. for an autotuner.  No counterpart in the
  * reference, whose deposit tiling is fixed (16x16 tiles, part2d_class.f03:99).
  * QP_TUNE_DEPOSIT_Q / QP_TUNE_AMJDP select one of the compiled shapes of the
- * charge / current deposit (rows per group, staging chunk, occupancy); shapes
- * differ only in how a cell's particles are split over lanes, so results
- * agree to rounding.  QP_TUNE_SPECULATE = 1 lets qp_slice_run enqueue the
+ * charge / current deposit (rows per group, fixed or, for the FP32 current
+ * deposit's default, chosen per tile from its particle count; staging chunk;
+ * occupancy); shapes differ only in how a cell's particles are split over
+ * lanes, so results agree to rounding.  QP_TUNE_SPECULATE = 1 lets qp_slice_run enqueue the
  * deposits of corrector pass l+1 before reading the residual of pass l (they
  * return at once if pass l converged; the default); 0 waits for each residual;
  * 2 also enqueues as many whole passes as the last slice used before the
