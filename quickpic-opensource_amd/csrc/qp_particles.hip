@@ -548,11 +548,12 @@ __device__ __forceinline__ R amjdp_one(const Fields6<R> &F, R fx, R fy, R x, R y
 // UQ: every particle's charge is quni (not staged).
 // RG = 0: the rows per group are chosen per tile, as 16 / T for the smallest
 // T (lanes per cell: 1, 2, 4, 8 or 16) with which the tile's particles,
-// spread evenly over its T groups, fit one staging chunk per group; a chunk
-// then mostly holds every cell of its group and all lanes compute (with a
-// fixed RG the chunks of a dense group hold only some of its cells, and the
-// lanes of the others wait).
-template <typename R, int RG, int CH, int WPE, bool PSI, bool UQ>
+// spread evenly over its T groups, stay within THR/4 staging chunks per
+// group; a chunk then holds more of its group's cells than with a fixed RG
+// in dense tiles (where the lanes of the cells a chunk does not hold wait),
+// and a sparse tile takes fewer, larger groups.  (Compiling each T as its
+// own path was measured slower: 443-444 against 454.6 slices/s at THR = 4.)
+template <typename R, int RG, int CH, int WPE, bool PSI, bool UQ, int THR = 4>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R *bxy,
         const R *psit, size_t cs, DepOut<R> out, R *halo, size_t halo_cs, R qbm, R dt, R dex,
@@ -570,7 +571,7 @@ k_amjdp(TileGeom g, Soa<R> s, const int *__restrict__ off, const R *fxy, const R
     int T = RG > 0 ? TILE / (RG > 0 ? RG : 1) : 1;
     if (RG == 0) {
         const int ntile = off[kb + TCELLS] - off[kb];
-        while (T < TILE && ntile > CH * T) T *= 2;
+        while (T < TILE && 4 * ntile > THR * CH * T) T *= 2;
     }
     const int rg = TILE / T;   // rows per group
     const int cl = tid / T, sub = tid % T;
@@ -1498,33 +1499,38 @@ static int deposit_border(qp_ctx *c, const DepOut<R> &out, int ncomp, int fold,
 // per tile), staging chunk CH and a waves-per-SIMD floor WPE (4 caps the
 // kernel at 128 VGPRs).  Variant 0 of each table is the default; qp_ctx_tune
 // selects another.  The FP32 current deposit's default chooses its rows per
-// group per tile: 451.7 against 434.8-437.2 slices/s on whole 1024^2, 16 ppc
-// steps with RG = 8 (variant 11), the deposit 661 against 720-726 ms per
-// step; in FP64 the per-tile choice was slower (1679 against 1605 ms), so
-// RG = 8 stays its default.
+// group per tile, keeping a group within two staging chunks (THR = 8):
+// 472.4-472.5 slices/s on whole 1024^2, 16 ppc steps at nb/n0 = 10, against
+// 464.0 within 1.5 chunks (variant 12), 455.0 within one (variant 14) and
+// 434.8-437.2 with a fixed RG = 8 (variant 11); FP64 likewise, 242.9-243.4
+// against 233.9-234.3 with a fixed RG = 8 (variant 11).
 struct DepShape {
     int rg, ch, wpe;
     int wave;          // current deposit: 1 k_amjdp_w (independent waves, staging
                        // chunk ch per wave); 2 k_amjdp_pf (next chunk prefetched
                        // into registers); 3 k_amjdp_dma (FP32: chunks copied into
                        // LDS by LDS-DMA, double-buffered)
+    int thr = 4;       // rg = 0: a group's share of the tile's particles is kept
+                       // at or below thr/4 of the chunk
 };
 template <typename R> struct DepTab;
 template <> struct DepTab<float> {
     static constexpr DepShape q[] = {{16, 4096, 1, 0}, {16, 1024, 1, 0}, {8, 2048, 1, 0},
                                      {16, 2048, 1, 0}};
-    static constexpr DepShape j[] = {{0, 1024, 4, 0}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
+    static constexpr DepShape j[] = {{0, 1024, 4, 0, 8}, {8, 1024, 1, 0}, {4, 1024, 4, 0},
                                      {16, 1024, 4, 0}, {4, 512, 4, 0},  {8, 512, 1, 0},
                                      {2, 256, 1, 1},   {2, 128, 1, 1},  {8, 1024, 1, 2},
-                                     {8, 512, 1, 2},   {8, 512, 1, 3},  {8, 1024, 4, 0}};
+                                     {8, 512, 1, 2},   {8, 512, 1, 3},  {8, 1024, 4, 0},
+                                     {0, 1024, 4, 0, 6}, {0, 1024, 4, 0, 10}, {0, 1024, 4, 0, 4}};
 };
 template <> struct DepTab<double> {
     static constexpr DepShape q[] = {{8, 1024, 1, 0}, {8, 2048, 1, 0}, {16, 1024, 1, 0},
                                      {8, 512, 1, 0}};
-    static constexpr DepShape j[] = {{8, 1024, 1, 0}, {4, 1024, 1, 0}, {4, 512, 1, 0},
+    static constexpr DepShape j[] = {{0, 1024, 1, 0, 8}, {4, 1024, 1, 0}, {4, 512, 1, 0},
                                      {8, 512, 1, 0},  {2, 512, 1, 0},  {16, 512, 1, 0},
                                      {2, 128, 1, 1},  {2, 64, 1, 1},   {8, 512, 1, 2},
-                                     {8, 256, 1, 2},   {8, 256, 1, 0},  {0, 1024, 1, 0}};
+                                     {8, 256, 1, 2},   {8, 256, 1, 0},  {8, 1024, 1, 0},
+                                     {0, 1024, 1, 0, 6}, {0, 1024, 1, 0, 10}, {0, 1024, 1, 0, 4}};
 };
 
 // kernels using more than 64 KB of dynamic LDS ask for it once
@@ -1795,11 +1801,11 @@ static int amjdp_v(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     } else {
         lds = (7 * (size_t)Stage<d.ch>::N + 8 * 289) * sizeof(R);
         if (sp->q_uniform)
-            kern = psi ? k_amjdp<R, d.rg, d.ch, d.wpe, true, true>
-                       : k_amjdp<R, d.rg, d.ch, d.wpe, false, true>;
+            kern = psi ? k_amjdp<R, d.rg, d.ch, d.wpe, true, true, d.thr>
+                       : k_amjdp<R, d.rg, d.ch, d.wpe, false, true, d.thr>;
         else
-            kern = psi ? k_amjdp<R, d.rg, d.ch, d.wpe, true, false>
-                       : k_amjdp<R, d.rg, d.ch, d.wpe, false, false>;
+            kern = psi ? k_amjdp<R, d.rg, d.ch, d.wpe, true, false, d.thr>
+                       : k_amjdp<R, d.rg, d.ch, d.wpe, false, false, d.thr>;
     }
     int rc = allow_lds(kern, lds, c);
     if (rc) return rc;
@@ -1836,6 +1842,9 @@ static int amjdp_out(qp_species *sp, const qp_field2d *ef, const qp_field2d *bf,
     case 9: rc = amjdp_v<R, 9>(sp, ef, bf, psit, dex, out, psi); break;
     case 10: rc = amjdp_v<R, 10>(sp, ef, bf, psit, dex, out, psi); break;
     case 11: rc = amjdp_v<R, 11>(sp, ef, bf, psit, dex, out, psi); break;
+    case 12: rc = amjdp_v<R, 12>(sp, ef, bf, psit, dex, out, psi); break;
+    case 13: rc = amjdp_v<R, 13>(sp, ef, bf, psit, dex, out, psi); break;
+    case 14: rc = amjdp_v<R, 14>(sp, ef, bf, psit, dex, out, psi); break;
     default: rc = amjdp_v<R, 0>(sp, ef, bf, psit, dex, out, psi); break;
     }
     if (rc) return rc;
