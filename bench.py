@@ -439,8 +439,9 @@ def main():
     ap.add_argument("--other-peaks", type=float, nargs="*", default=[1.0, 20.0],
                     help="nb/n0 of the other legs (one warm-up and one timed step each)")
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--fp32-seed", type=int, default=1, choices=(0, 1),
-                    help="corrector seed of the FP32 leg: 0 previous slice, 1 extrapolated")
+    ap.add_argument("--fp32-seed", type=int, default=1, choices=(0, 1, 3),
+                    help="corrector seed of the FP32 leg: 0 previous slice, 1 linear, "
+                         "3 quadratic extrapolation")
     ap.add_argument("--fp32-mix", type=int, default=0, choices=(0, 1),
                     help="corrector iteration of the FP32 leg: 0 plain, 1 Anderson-mixed")
     ap.add_argument("--speculate", type=int, default=1, choices=(0, 1, 2),
@@ -530,10 +531,11 @@ def main():
                            f"the step at slice {n0 + 1} (a plasma electron crossing more than "
                            "one tile)")
                         + "; FP32 corrector seed: "
-                        + {0: "previous slice", 1: "linear extrapolation"}[args.fp32_seed]
+                        + {0: "previous slice", 1: "linear extrapolation",
+                           3: "quadratic extrapolation"}[args.fp32_seed]
                         + {0: "", 1: ", Anderson-mixed passes"}[args.fp32_mix]
                         + "; FP64 leg: previous slice (plain)",
-            "fp32_seed": {0: "previous", 1: "extrapolated"}[args.fp32_seed],
+            "fp32_seed": {0: "previous", 1: "extrapolated", 3: "quadratic"}[args.fp32_seed],
             "fp32_corrector": {0: "plain", 1: "anderson1"}[args.fp32_mix],
             "speculative_deposits": bool(args.speculate),
             "grid": [2 ** args.ind, 2 ** args.ind, 2 ** args.indz],

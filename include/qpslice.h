@@ -117,7 +117,9 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * QP_TUNE_SEED = 1 seeds the corrector of a slice run
  * with a residual tolerance by linear extrapolation 2 B(xi-1) - B(xi-2)
  * instead of the reference's B(xi-1) (default 0); QP_TUNE_SEED = 2 keeps the
- * B_perp a caller wrote between qp_slice_prepare and qp_slice_finish.  The
+ * B_perp a caller wrote between qp_slice_prepare and qp_slice_finish;
+ * QP_TUNE_SEED = 3 extrapolates quadratically, 3 B(xi-1) - 3 B(xi-2) +
+ * B(xi-3) (linearly while only two slices are recorded).  The
  * seed changes only how many passes reach the tolerance; fixed-pass runs
  * (tol <= 0) use the reference's seed whatever the knob.
  * QP_TUNE_FFT = 1 folds the transpose between the x and y passes of every

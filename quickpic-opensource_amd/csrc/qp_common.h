@@ -269,11 +269,14 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy);
 // converged flag).  With qp the residual of pass l is written to h_res and,
 // when tol > 0 and it is below tol, the converged flag is set.
 // seed (predictor only): 1 records B_perp in ehist, 2 also extrapolates
-// epw <- 2 epw - ehist first (the corrector seed of QP_TUNE_SEED = 1); with
-// seed_hist (captured slices) the predictor reads the count of recorded
-// slices from device memory and extrapolates once it is 2 or more
+// epw <- 2 epw - ehist first (the corrector seed of QP_TUNE_SEED = 1), 3
+// (with ehist2, which keeps the slice before) 3 epw - 3 B(j-2) + B(j-3)
+// (QP_TUNE_SEED = 3); with seed_hist (captured slices) the predictor
+// reads the count of recorded slices from device memory and extrapolates
+// once it is 2 or more (quadratically once it is 3 or more)
 int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int seed = 0,
-               qp_field2d *ehist = nullptr, const int *seed_hist = nullptr);
+               qp_field2d *ehist = nullptr, const int *seed_hist = nullptr,
+               qp_field2d *ehist2 = nullptr);
 // Anderson mixing (depth 1) of the corrector iterate after pass l (f.gprev,
 // f.fprev set; the pass was run by fused_pass with them): unless pass l has
 // converged, epw <- g_l - gamma (g_l - g_{l-1}) and B_perp reassembled; mix =

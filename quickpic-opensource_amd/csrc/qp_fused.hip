@@ -39,22 +39,27 @@ __global__ void k_psi_fxy(KGrid g, const R *__restrict__ ffg, const R *__restric
 // corrector's stopping test.  With MIX the change f = epw_new - epw_old is
 // also kept in fprev, and <f, f - f_prev> and |f - f_prev|^2 (f_prev = 0 on
 // the first pass, hist = 0) are summed per block into mixp for k_mix.
-// The predictor (no QP) also applies the corrector seed by linear
-// extrapolation in xi to each mode first: seed = 1 keeps epw in ehist, seed
-// = 2 also replaces epw by 2 epw - ehist.  Only the iteration count to a
-// given residual depends on the seed.
+// The predictor (no QP) also applies the corrector seed by extrapolation in
+// xi to each mode first: seed = 1 keeps epw (B(j-1)) in ehist, seed = 2 also
+// replaces epw by 2 epw - ehist; with ehist2 (quadratic), ehist2 keeps the
+// slice before and seed = 3 replaces epw by 3 epw - 3 B(j-2) + B(j-3).  Only the iteration count to a given residual
+// depends on the seed.
 template <typename R, bool QP, bool MIX>
 __global__ void __launch_bounds__(256)
 k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *__restrict__ cu,
        R *dcu, const R *__restrict__ amu, R *epw, const R *__restrict__ epwb, R *psit, R *bxyz,
        R aa, R idex, R dex, double *partial, int *flags, const int *skip, R *fprev, int hist,
-       double *mixp, R *ehist, int seed, const int *seed_hist)
+       double *mixp, R *ehist, int seed, const int *seed_hist, R *ehist2)
 {
     __shared__ double s_num[256], s_den[256], s_ma[MIX ? 256 : 1], s_mb[MIX ? 256 : 1];
     if (skip && *skip) return;
     // captured slices: the slices recorded in ehist so far come from device
     // memory (2 or more: extrapolate; fewer: only record)
-    if (seed && seed_hist) seed = *seed_hist >= 2 ? 2 : 1;
+    if (seed && seed_hist) {
+        const int n = *seed_hist;
+        seed = n >= 3 ? 3 : (n >= 2 ? 2 : 1);
+    }
+    if (seed == 3 && !ehist2) seed = 2;
     if (!QP && blockIdx.x == 0 && threadIdx.x < FLAG_WORDS) flags[threadIdx.x] = 0;
     double num = 0.0, den = 0.0, ma = 0.0, mb = 0.0;
     const size_t cs = g.cs;
@@ -71,11 +76,23 @@ k_pass(KGrid g, const R *__restrict__ ffg, const R *__restrict__ ffs, const R *_
         R e1 = epw[o], e2 = epw[cs + o];
         if (!QP && seed) {
             const R h1 = ehist[o], h2 = ehist[cs + o];
+            R g1 = R(0), g2 = R(0);
+            if (ehist2) {
+                g1 = ehist2[o];
+                g2 = ehist2[cs + o];
+                ehist2[o] = h1;
+                ehist2[cs + o] = h2;
+            }
             ehist[o] = e1;
             ehist[cs + o] = e2;
             if (seed == 2) {
                 e1 = R(2) * e1 - h1;
                 e2 = R(2) * e2 - h2;
+                epw[o] = e1;
+                epw[cs + o] = e2;
+            } else if (seed == 3) {
+                e1 = (R(3) * e1 - R(3) * h1) + g1;
+                e2 = (R(3) * e2 - R(3) * h2) + g2;
                 epw[o] = e1;
                 epw[cs + o] = e2;
             }
@@ -310,7 +327,7 @@ int fused_psi_fxy(qp_field2d *q, qp_field2d *psi, qp_field2d *fxy)
 
 template <typename R>
 static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, int seed,
-                  qp_field2d *ehist, const int *seed_hist)
+                  qp_field2d *ehist, const int *seed_hist, qp_field2d *ehist2)
 {
     qp_ctx *c = f.cu->ctx;
     double *part = c->red.as<double>();
@@ -327,7 +344,8 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, i
                        (R)dex, part, c->flags.as<int>(), qp ? c->skip : nullptr,
                        mix ? f.fprev->ks.as<R>() : nullptr, l >= 2 ? 1 : 0,
                        part + MIX_PARTIALS, seed ? ehist->ks.as<R>() : nullptr, qp ? 0 : seed,
-                       qp ? nullptr : seed_hist);
+                       qp ? nullptr : seed_hist,
+                       seed && ehist2 ? ehist2->ks.as<R>() : nullptr);
     }
     QP_LAUNCH_CHECK();
     if (qp) {
@@ -343,12 +361,13 @@ static int pass_t(const PassFields &f, double dex, bool qp, int l, double tol, i
 }
 
 int fused_pass(const PassFields &f, double dex, bool qp, int l, double tol, int seed,
-               qp_field2d *ehist, const int *seed_hist)
+               qp_field2d *ehist, const int *seed_hist, qp_field2d *ehist2)
 {
     QP_REQUIRE(!seed || (ehist && !qp), QP_ERR_ARG, "fused_pass: seed without history");
+
     return f.cu->ctx->prec == QP_FP64
-               ? pass_t<double>(f, dex, qp, l, tol, seed, ehist, seed_hist)
-               : pass_t<float>(f, dex, qp, l, tol, seed, ehist, seed_hist);
+               ? pass_t<double>(f, dex, qp, l, tol, seed, ehist, seed_hist, ehist2)
+               : pass_t<float>(f, dex, qp, l, tol, seed, ehist, seed_hist, ehist2);
 }
 
 template <typename R>

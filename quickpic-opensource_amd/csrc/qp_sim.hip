@@ -40,6 +40,7 @@ struct qp_sim {
     qp_field2d *f[QP_F_COUNT];
     qp_field3d *bexyz, *bbxyz;
     qp_field2d *epw_hist;   // B_perp of the slice before the last (extrapolated seed)
+    qp_field2d *epw_hist2;  // and of the one before it (quadratic seed)
     qp_field2d *gprev, *fprev;   // last pass's result and change (mixed corrector)
     int hist_slices;        // slices of this step recorded in the history
     int prepared;           // slice whose first part has run (0: none)
@@ -111,6 +112,7 @@ int qp_sim_create(qp_ctx *c, qp_species *const *sp, int nsp, qp_beam *const *bm,
     if (!rc) rc = qp_field3d_create(c, 3, &s->bexyz);
     if (!rc) rc = qp_field3d_create(c, 3, &s->bbxyz);
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->epw_hist);
+    if (!rc) rc = qp_field2d_create(c, 2, 1, &s->epw_hist2);
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->gprev);
     if (!rc) rc = qp_field2d_create(c, 2, 1, &s->fprev);
     if (!rc) rc = s->ctl.alloc(4 * sizeof(int));
@@ -127,6 +129,7 @@ int qp_sim_destroy(qp_sim *s)
     qp_field3d_destroy(s->bexyz);
     qp_field3d_destroy(s->bbxyz);
     qp_field2d_destroy(s->epw_hist);
+    qp_field2d_destroy(s->epw_hist2);
     qp_field2d_destroy(s->gprev);
     qp_field2d_destroy(s->fprev);
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
@@ -301,14 +304,17 @@ int qp_slice_finish(qp_sim *s, int j, int iter_max, double tol, int *iters_used,
         pc.gprev = s->gprev;
         pc.fprev = s->fprev;
     }
-    // the extrapolated corrector seed is applied by the predictor's kernel
+    // the extrapolated corrector seed is applied by the predictor's kernel:
+    // linear (QP_TUNE_SEED = 1) or quadratic (3) in xi
     int seed = 0;
-    if (adaptive && c->tune[QP_TUNE_SEED] == 1) {
-        seed = s->hist_slices >= 2 ? 2 : 1;
+    const bool quad = c->tune[QP_TUNE_SEED] == 3;
+    if (adaptive && (c->tune[QP_TUNE_SEED] == 1 || quad)) {
+        seed = s->hist_slices >= 3 ? 3 : (s->hist_slices >= 2 ? 2 : 1);
         ++s->hist_slices;
     }
     RC(fused_pass(pf, dex, false, 0, 0.0, seed, s->epw_hist,
-                  graph && seed ? s->ctl.as<int>() + 1 : nullptr));
+                  graph && seed ? s->ctl.as<int>() + 1 : nullptr,
+                  quad ? s->epw_hist2 : nullptr));
     RC(fft_k2r_batch(c, inv, inv_kinds, 2));
     psit->state = bxyz->state = 0;
     // corrector passes (:676-710)

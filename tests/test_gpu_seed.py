@@ -88,6 +88,33 @@ def test_any_seed_converges_to_the_same_fields():
         assert rel(a, b) < 1e-4
 
 
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_extrapolated_seeds_converge_to_the_same_fields(precision):
+    """The linear (QP_TUNE_SEED = 1) and quadratic (3) extrapolations in xi
+    only move where the corrector starts: every slice of a dense driver
+    reaches the tolerance from each seed, and the fields agree with the
+    reference's previous-slice seed within the tolerance's reach; the
+    passes of each seed are printed."""
+    banner(f"extrapolated seeds {precision}")
+    prec = getattr(qps, precision)
+    runs = {}
+    for seed in (0, 1, 3):
+        ctx, sim = make(prec)
+        ctx.tune(qps.TUNE_SEED, seed)
+        passes = []
+        for j in range(1, 25):
+            it, r = sim.slice(j, iter_max=30, tol=TOL)
+            assert it < 30 and r < TOL, (seed, j)
+            passes.append(it)
+        runs[seed] = (passes, fields(sim))
+        print(f"seed {seed}: passes {passes} (sum {sum(passes)})", flush=True)
+    for seed in (1, 3):
+        for a, b in zip(runs[seed][1], runs[0][1]):
+            assert rel(a, b) < 1e-4, seed
+    # the quadratic seed extrapolates once three slices are recorded
+    assert runs[3][0][:3] == runs[1][0][:3]
+
+
 def test_features_are_block_means_of_the_named_planes():
     banner("seed features")
     ctx, sim = make()
