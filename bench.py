@@ -112,6 +112,7 @@ def run_leg(args, precision, peak, nwarm, nsteps, nslices, beam_part=None, break
         ctx.tune(qps.TUNE_SPECULATE, args.speculate)
         ctx.tune(qps.TUNE_FFT, args.fft if precision == qps.FP32 else -1)
         ctx.tune(qps.TUNE_AMJDP, args.amjdp)
+        ctx.tune(qps.TUNE_DEPOSIT_Q, args.deposit_q)
         t0 = time.time()
         sp = qps.Species(ctx, ppc=(args.ppc, args.ppc))
         bm = qps.Beam(ctx, center=(0.0, 0.0, 2.0), sigma=(args.sigma_r, args.sigma_r, 0.5),
@@ -451,6 +452,8 @@ def main():
                          "between them, 1 column-pair passes, -1 the library's default")
     ap.add_argument("--amjdp", type=int, default=-1,
                     help="launch shape of the current deposit (QP_TUNE_AMJDP), -1 the default")
+    ap.add_argument("--deposit-q", type=int, default=-1,
+                    help="launch shape of the charge deposit (QP_TUNE_DEPOSIT_Q), -1 the default")
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")
