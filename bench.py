@@ -447,9 +447,10 @@ def main():
                     help="corrector iteration of the FP32 leg: 0 plain, 1 Anderson-mixed")
     ap.add_argument("--speculate", type=int, default=1, choices=(0, 1, 2),
                     help="enqueue the next corrector pass's deposits before reading a residual")
-    ap.add_argument("--fft", type=int, default=-1, choices=(-1, 0, 1),
-                    help="transform form of the FP32 leg: 0 row passes with a transpose "
-                         "between them, 1 column-pair passes, -1 the library's default")
+    ap.add_argument("--fft", type=int, default=-1, choices=(-1, 0, 1, 2, 3, 4, 5, 6),
+                    help="transform form of the FP32 leg (QP_TUNE_FFT): 0 row passes with a "
+                         "transpose between them, 1 column-pair passes, 5 / 6 column pairs "
+                         "only for k -> real / real -> k, -1 the library's default")
     ap.add_argument("--amjdp", type=int, default=-1,
                     help="launch shape of the current deposit (QP_TUNE_AMJDP), -1 the default")
     ap.add_argument("--deposit-q", type=int, default=-1,

@@ -126,8 +126,9 @@ int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,
  * transform into the y pass (column-pair reads / writes) instead of a
  * separate LDS-tiled transpose (the FP64 default).  0 (the FP32 default),
  * 2, 3 and 4 keep the transpose and pad the transforms' LDS by one point per
- * 16 points in FP32 or 8 in FP64 (0), or per 8, 32 or 4 points (2, 3, 4).
- * Results are identical.
+ * 16 points in FP32 or 8 in FP64 (0), or per 8, 32 or 4 points (2, 3, 4);
+ * 5 and 6 fold the transpose only in the k -> real or only in the real -> k
+ * direction.  Results are identical.
  * QP_TUNE_PUSH selects how the plasma push re-sorts the particles: 0 (default)
  * counts the destination bins in a first pass that stores nothing per
  * particle, then pushes again and stores every particle at its sorted place;

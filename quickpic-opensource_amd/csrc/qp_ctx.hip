@@ -214,7 +214,7 @@ using namespace qp;
 // transforms read / write the y pass as column pairs (no transpose kernel;
 // identical results, 8 % less transform time at 1024^2 in FP64, 5 % more in
 // FP32)
-static const int TUNE_VALUES[QP_TUNE_COUNT] = {4, 15, 3, 4, 5, 2, 2, 2};
+static const int TUNE_VALUES[QP_TUNE_COUNT] = {4, 15, 3, 4, 7, 2, 2, 2};
 static const int TUNE_DEFAULT[QP_TUNE_COUNT] = {0, 0, 1, 0, 0, 0, 0, 0};
 
 namespace qp {

@@ -465,7 +465,9 @@ static int r2k_core(qp_ctx *c, R *const *rs, R *const *ks, int nc, int mx, int m
     const double ani = 1.0 / ((double)c->nx * (double)c->ny);
     int rc = rowfft0<R>(c, a, nc, c->ny + 1, c->nx, ldr, ldr, mx, ani, twx, -1);
     if (rc) return rc;
-    if (c->tune[QP_TUNE_FFT] == 1)   // y pass straight from the column pairs of the x pass
+    // y pass straight from the column pairs of the x pass (QP_TUNE_FFT 1, or 6:
+    // this direction only)
+    if (c->tune[QP_TUNE_FFT] == 1 || c->tune[QP_TUNE_FFT] == 6)
         return rowfft<R, 1, pad_shift<R>()>(c, b, nc, c->nx + 1, c->ny, ldr, ldk, my, 1.0,
                                             twy, -1);
     rc = transpose<R>(c, b, nc, ldr, ldk, c->ny + 1, c->nx + 1);
@@ -492,7 +494,9 @@ static int k2r_core(qp_ctx *c, R *const *rs, R *const *ks, int nc, int mx, int m
         yc.dst[i] = rs[i];
     }
     int rc;
-    if (c->tune[QP_TUNE_FFT] == 1) {   // y pass written as column pairs of rs
+    // y pass written as column pairs of rs (QP_TUNE_FFT 1, or 5: this
+    // direction only)
+    if (c->tune[QP_TUNE_FFT] == 1 || c->tune[QP_TUNE_FFT] == 5) {
         rc = rowfft<R, 2, pad_shift<R>()>(c, yc, nc, c->nx + 1, c->ny, ldk, ldr, my, 2.0, twy,
                                           -1);
     } else {
