@@ -143,8 +143,11 @@ def run_leg(args, precision, peak, nwarm, nsteps, nslices, beam_part=None, break
             else:
                 run_step(sim, args, nz)
             log(f"[bench] {name} nb/n0={peak:g}: warm-up step {w + 1} {time.time() - t0:.2f}s")
-        # timed steps: HIP events on the current deposit's launches only
-        ctx.profile(True, classes=[qps.PROF_AMJDP])
+        # timed steps: HIP events on one current-deposit launch in
+        # --events-every, spread evenly over the steps' passes and slices
+        if args.events_every > 0:
+            ctx.profile(True, classes=[qps.PROF_AMJDP])
+            ctx.profile_every(args.events_every)
         passes = []
         ctx.sync()
         t0 = time.perf_counter()
@@ -154,6 +157,7 @@ def run_leg(args, precision, peak, nwarm, nsteps, nslices, beam_part=None, break
         elapsed = time.perf_counter() - t0
         am_ms, am_n, am_b = ctx.profile_get(qps.PROF_AMJDP)
         ctx.profile(False)
+        ctx.profile_every(1)
         res = {"slices_per_s": nz * nsteps / elapsed, "ms_per_step": elapsed / nsteps * 1e3,
                "passes_mean": float(np.mean(passes)), "passes_max": int(np.max(passes)),
                "np_plasma": sp.np, "np_beam": bm.np, "wake": wake, "beam_energy": energy,
@@ -455,6 +459,10 @@ def main():
                     help="launch shape of the current deposit (QP_TUNE_AMJDP), -1 the default")
     ap.add_argument("--deposit-q", type=int, default=-1,
                     help="launch shape of the charge deposit (QP_TUNE_DEPOSIT_Q), -1 the default")
+    ap.add_argument("--events-every", type=int, default=8,
+                    help="time one current-deposit launch in this many during the timed "
+                         "steps, spread evenly over them (0: none, no roofline); every "
+                         "launch's events cost the timed steps 2-3 %%")
     ap.add_argument("--iter-max", type=int, default=20)
     ap.add_argument("--no-fp64", dest="fp64", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false")

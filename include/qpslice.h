@@ -96,6 +96,11 @@ int qp_ctx_profile(qp_ctx *ctx, int enable);   /* enabling also resets */
 /* the same for the classes in mask only (bit k = class k; 0 = off), so a
  * timed run can carry events on one kernel class at little cost */
 int qp_ctx_profile_classes(qp_ctx *ctx, unsigned mask);
+/* Time one launch in `every` of each enabled class (default 1: all): the
+ * launches of a long run are sampled evenly, and the events' own cost stays
+ * small.  The counts and bytes qp_ctx_profile_get returns are those of the
+ * timed launches. */
+int qp_ctx_profile_every(qp_ctx *ctx, int every);
 /* bytes: the algorithmic bytes of those launches (operands read or written
  * once: particle records and field planes), the figure a roofline uses */
 int qp_ctx_profile_get(qp_ctx *ctx, int kind, double *total_ms, int64_t *count,

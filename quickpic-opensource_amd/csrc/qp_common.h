@@ -124,6 +124,8 @@ struct qp_ctx {
     size_t max_lds;               // LDS bytes one workgroup may use
     // kernel-class timing (qp_ctx_profile)
     unsigned prof_on;     // bit k: launches of class k are timed
+    int prof_every;       // time one launch in prof_every of a class (0, 1: all)
+    int64_t prof_seen[QP_PROF_COUNT];   // launches of each class since the reset
     int prof_tag;         // tag of the launches being enqueued (prof_discard)
     int prof_epoch;       // slices run so far: tags are unique per slice
     std::vector<qp::ProfEv> prof_pending;

@@ -161,6 +161,7 @@ static hipEvent_t pool_get(qp_ctx *c)
 
 int prof_begin(qp_ctx *c, int kind, double bytes)
 {
+    if (c->prof_every > 1 && c->prof_seen[kind]++ % c->prof_every != 0) return -1;
     ProfEv ev{pool_get(c), pool_get(c), kind, bytes, c->prof_tag};
     if (!ev.a || !ev.b) return -1;
     if (hipEventRecord(ev.a, c->stream) != hipSuccess) return -1;
@@ -320,8 +321,17 @@ int qp_ctx_profile_classes(qp_ctx *c, unsigned mask)
         c->prof_ms[k] = 0.0;
         c->prof_n[k] = 0;
         c->prof_bytes[k] = 0.0;
+        c->prof_seen[k] = 0;
     }
     c->prof_on = mask & ((1u << QP_PROF_COUNT) - 1u);
+    return QP_OK;
+}
+
+int qp_ctx_profile_every(qp_ctx *c, int every)
+{
+    QP_REQUIRE(c && every >= 1, QP_ERR_ARG, "qp_ctx_profile_every: bad argument");
+    c->prof_every = every;
+    for (int k = 0; k < QP_PROF_COUNT; ++k) c->prof_seen[k] = 0;
     return QP_OK;
 }
 

@@ -158,6 +158,10 @@ class Ctx:
             mask = sum(1 << k for k in classes) if enable else 0
             _call(lib().qp_ctx_profile_classes, self.h, C.c_uint(mask))
 
+    def profile_every(self, every):
+        """Times one launch in `every` of each enabled class (1: all)."""
+        _call(lib().qp_ctx_profile_every, self.h, int(every))
+
     def profile_get(self, kind):
         """(total milliseconds, launches, algorithmic bytes) of one kernel
         class (PROF_*)."""

@@ -260,6 +260,34 @@ def test_profiler_counts_one_deposit_per_pass(speculate):
         ctx.close()
 
 
+def test_profiler_samples_one_launch_in_k():
+    """With qp_ctx_profile_every(k) one launch in k of a class is timed: the
+    1st, (k+1)th, ... of a run, so a run of n current-deposit launches
+    (speculation off: one per pass) yields ceil(n / k) timings, and their
+    mean launch time is that of the run's launches."""
+    banner("profiler sampling")
+    ctx = qps.Ctx(IND, IND, INDZ, box=BOX, precision=qps.FP32)
+    try:
+        ctx.tune(qps.TUNE_SPECULATE, 0)
+        sp = qps.Species(ctx, ppc=(2, 2))
+        bm = qps.Beam(ctx, center=(0.0, 0.0, 1.5), sigma=(0.5, 0.5, 0.5), np_=(32, 32, 64),
+                      peak=10.0, dt=10.0)
+        gsim = qps.Sim(ctx, [sp], [bm], iter_=2)
+        gsim.begin_step()
+        kind = qps.PROF_NAMES.index("amjdp")
+        ctx.profile(True, classes=[kind])
+        ctx.profile_every(3)
+        passes = [gsim.slice(j, iter_max=20, tol=1e-6)[0] for j in range(1, 21)]
+        t, n, b = ctx.profile_get(kind)
+        ctx.profile(False)
+        ctx.profile_every(1)
+        print("passes", sum(passes), "timed", n, flush=True)
+        assert n == -(-sum(passes) // 3)
+        assert t > 0 and b > 0
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("precision", ["FP32", "FP64"])
 @pytest.mark.parametrize("push", [0, 1])
 def test_move_check_reports_a_tile_jump(precision, push):
