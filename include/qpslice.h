@@ -377,6 +377,9 @@ int qp_beam_create_twiss(qp_ctx *ctx, const qp_beam_twiss_params *p, double dt,
 int qp_beam_destroy(qp_beam *b);
 int qp_beam_qdp(qp_beam *b);                                       /* :111 */
 int qp_beam_qdpcopy(qp_beam *b, qp_field2d *q, int slice);         /* :136 */
+/* drops the particles that leave [1, n-1) and leaves the survivors sorted by
+ * cell (z, then y, then x); the reference fills holes from the end instead,
+ * so the survivor set is the same and the order is not */
 int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf,
                  double dex, double dez);                          /* :155 */
 int qp_beam_count(const qp_beam *b, int64_t *np);

@@ -24,7 +24,10 @@ struct qp_beam {
     int64_t np, npmax;
     qp::DevBuf a[7], b[7];       // x, y, z, px, py, pz, q (double)
     qp::DevBuf alive, pos, scan_tmp, moments;
-    size_t scan_bytes;
+    qp::DevBuf key[2], perm[2];  // push: cell keys and particle indices, before and after the sort
+    size_t scan_bytes;           // temporary bytes of the scan and the sort
+    int key_shift, key_bits;     // cell key = linear cell index >> key_shift, below 2^key_bits
+    unsigned key_drop;           // key of a dropped particle (sorts last)
     qp::DevBuf q3;               // int64 [nz+1][ny+1][ldr]
 };
 
@@ -284,36 +287,69 @@ static int64_t load_beam_twiss(qp_ctx *c, const qp_beam_twiss_params &bp, uint64
     return npp;
 }
 
-// PGPOST32L (part3d_lib77.f:554-628), int64 fixed-point accumulation
-__global__ void k_beam_deposit(int64_t np, const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, const double *__restrict__ q,
-                               unsigned long long *q3, int ld, size_t pl, int nx, int ny, int nz)
+// PGPOST32L (part3d_lib77.f:554-628), int64 fixed-point accumulation.
+// Particles of one wavefront that sit in the same cell on neighbouring lanes
+// (the push leaves the beam sorted by cell) are summed first, in int64 across
+// the wavefront, and the first lane of each such run adds the sums with one
+// atomic per node. Integer sums make the charge the same, bit for bit, as one
+// atomic per particle and node, whatever the particle order.
+__global__ __launch_bounds__(256) void k_beam_deposit(
+    int64_t np, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ q, unsigned long long *q3, int ld,
+    size_t pl, int nx, int ny, int nz)
 {
-    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= np) return;
-    const double px = x[j], py = y[j], pz = z[j], qm = q[j];
-    // a particle outside the grid has no nodes to deposit to
-    if (!(px >= 0.0 && px < (double)nx && py >= 0.0 && py < (double)ny && pz >= 0.0 &&
-          pz < (double)nz))
-        return;
-    int nn = (int)px, mm = (int)py, ll = (int)pz;
-    double dxp = qm * (px - (double)nn);
-    double dyp = py - (double)mm;
-    double dzp = pz - (double)ll;
-    double amx = qm - dxp;
-    double amy = 1. - dyp;
-    double dx1 = dxp * dyp;
-    dyp = amx * dyp;
-    amx = amx * amy;
-    double amz = 1. - dzp;
-    amy = dxp * amy;
-    unsigned long long *b = q3 + (size_t)ll * pl + (size_t)mm * ld + nn;
-    const double w[8] = {amx * amz, amy * amz, dyp * amz, dx1 * amz,
-                         amx * dzp, amy * dzp, dyp * dzp, dx1 * dzp};
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    constexpr unsigned long long NONE = ~0ull;
+    unsigned long long cell = NONE;
+    long long w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (j < np) {
+        const double px = x[j], py = y[j], pz = z[j], qm = q[j];
+        // a particle outside the grid has no nodes to deposit to
+        if (px >= 0.0 && px < (double)nx && py >= 0.0 && py < (double)ny && pz >= 0.0 &&
+            pz < (double)nz) {
+            int nn = (int)px, mm = (int)py, ll = (int)pz;
+            double dxp = qm * (px - (double)nn);
+            double dyp = py - (double)mm;
+            double dzp = pz - (double)ll;
+            double amx = qm - dxp;
+            double amy = 1. - dyp;
+            double dx1 = dxp * dyp;
+            dyp = amx * dyp;
+            amx = amx * amy;
+            double amz = 1. - dzp;
+            amy = dxp * amy;
+            cell = (size_t)ll * pl + (size_t)mm * ld + nn;
+            const double f[8] = {amx * amz, amy * amz, dyp * amz, dx1 * amz,
+                                 amx * dzp, amy * dzp, dyp * dzp, dx1 * dzp};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w[k] = __double2ll_rn(f[k] * BEAM_FIX);
+        }
+    }
+    // a run of equal cells starts where the lane before holds another cell
+    const unsigned long long prev = __shfl_up(cell, 1, 64);
+    const bool head = lane == 0 || prev != cell;
+    const unsigned long long heads = __ballot(head);
+    if (heads != ~0ull) {
+        // run number of each lane, then suffix sums inside each run: after
+        // the step of stride d a lane holds its run's sum over [lane, lane + 2d)
+        const int run = __popcll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int orun = __shfl_down(run, d, 64);
+            const bool take = lane + d < 64 && orun == run;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const long long o = __shfl_down(w[k], d, 64);
+                if (take) w[k] += o;
+            }
+        }
+    }
+    if (!head || cell == NONE) return;
+    unsigned long long *b = q3 + cell;
     const size_t o[8] = {0, 1, (size_t)ld, (size_t)ld + 1, pl, pl + 1, pl + ld, pl + ld + 1};
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        atomicAdd(b + o[k], (unsigned long long)__double2ll_rn(w[k] * BEAM_FIX));
+    for (int k = 0; k < 8; ++k) atomicAdd(b + o[k], (unsigned long long)w[k]);
 }
 
 // qdpcopy / copyadd of one xi plane (ufield3d_class.f03:460-490) from the
@@ -337,21 +373,40 @@ __global__ void k_beam_slice_add(R *q2, const long long *q3, size_t n, int assig
     }
 }
 
+// Sort key of a beam particle: its cell's linear index (z, then y, then x)
+// shifted right by `shift` bits so that it fits 32 bits; dropped particles
+// carry `drop`, above every cell's key.
+struct CellKey {
+    int nx, ny, shift;
+    unsigned drop;
+    __device__ unsigned of(double X, double Y, double Z) const
+    {
+        const unsigned long long c =
+            ((unsigned long long)(int)Z * (unsigned)ny + (unsigned)(int)Y) * (unsigned)nx +
+            (unsigned)(int)X;
+        return (unsigned)(c >> shift);
+    }
+};
+
 // PGBPUSH32L_QP (part3d_lib77.f:630-786) with ipbc = 1 and cofd = 0; flags
-// the particles that leave [1, n-1) in any direction.
+// the particles that leave [1, n-1) in any direction, and writes each
+// particle's sort key (its new cell) with its index beside it.
 template <typename R>
 __global__ void k_beam_push(int64_t np, double *x, double *y, double *z, double *px, double *py,
                             double *pz, const R *__restrict__ e3, const R *__restrict__ b3,
                             size_t vol, int ld, size_t pl, double qtmh, double dtcx, double dtcz,
-                            double rx, double ry, double rz, int *alive)
+                            double rx, double ry, double rz, int *alive, CellKey ck,
+                            unsigned *key, unsigned *idx)
 {
     int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= np) return;
+    idx[j] = (unsigned)j;
     double X = x[j], Y = y[j], Z = z[j];
     // outside the grid nothing can be gathered: dropped without touching the
     // fields (the loader and the push keep every particle in [1, n-1))
     if (!(X >= 0.0 && X < rx + 1.0 && Y >= 0.0 && Y < ry + 1.0 && Z >= 0.0 && Z < rz + 1.0)) {
         alive[j] = 0;
+        key[j] = ck.drop;
         return;
     }
     int nn = (int)X, mm = (int)Y, ll = (int)Z;
@@ -391,19 +446,21 @@ __global__ void k_beam_push(int64_t np, double *x, double *y, double *z, double 
                   (nzp >= rz));
     if (live) { x[j] = nxp; y[j] = nyp; z[j] = nzp; }
     alive[j] = live ? 1 : 0;
+    key[j] = live ? ck.of(nxp, nyp, nzp) : ck.drop;
 }
 
 struct Ptr7 {
     double *p[7];
 };
 
-__global__ void k_compact(int64_t np, const int *alive, const int *pos, Ptr7 src, Ptr7 dst)
+// survivor i of the cell-sorted beam is particle perm[i] (i < *nlive)
+__global__ void k_permute(int64_t np, const int *nlive, const unsigned *perm, Ptr7 src, Ptr7 dst)
 {
-    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= np || !alive[j]) return;
-    int d = pos[j];
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= np || i >= *nlive) return;
+    const unsigned s = perm[i];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) dst.p[k][d] = src.p[k][j];
+    for (int k = 0; k < 7; ++k) dst.p[k][i] = src.p[k][s];
 }
 
 __global__ void k_beam_moments(int64_t np, const double *px, const double *py, const double *pz,
@@ -448,12 +505,28 @@ static int beam_alloc(qp_beam *b, int64_t n)
     }
     int rc = b->alive.alloc((size_t)(n + 1) * 4);
     if (!rc) rc = b->pos.alloc((size_t)(n + 1) * 4);
+    for (int k = 0; k < 2 && !rc; ++k) {
+        rc = b->key[k].alloc((size_t)(n + 1) * 4);
+        if (!rc) rc = b->perm[k].alloc((size_t)(n + 1) * 4);
+    }
     if (!rc) rc = b->moments.alloc(sizeof(double) * 2 * 1024 + 64);
     if (!rc) rc = b->q3.alloc((size_t)(c->nz + 1) * c->rplane * 8);
     if (rc) return rc;
-    b->scan_bytes = 0;
-    QP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b->scan_bytes, b->alive.as<int>(),
-                                            b->pos.as<int>(), (int)(n + 1), c->stream));
+    // the smallest shift that puts every cell's key below the drop key < 2^32
+    const unsigned long long ncell = (unsigned long long)c->nx * c->ny * c->nz;
+    b->key_shift = 0;
+    while (((ncell - 1) >> b->key_shift) >= 0xFFFFFFFFull) ++b->key_shift;
+    b->key_drop = (unsigned)(((ncell - 1) >> b->key_shift) + 1);
+    b->key_bits = 1;
+    while (b->key_bits < 32 && (b->key_drop >> b->key_bits) != 0) ++b->key_bits;
+    size_t scan = 0, sort = 0;
+    QP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan, b->alive.as<int>(), b->pos.as<int>(),
+                                            (int)(n + 1), c->stream));
+    QP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort, b->key[0].as<unsigned>(),
+                                              b->key[1].as<unsigned>(), b->perm[0].as<unsigned>(),
+                                              b->perm[1].as<unsigned>(), (int)n, 0, b->key_bits,
+                                              c->stream));
+    b->scan_bytes = std::max(scan, sort);
     return b->scan_tmp.alloc(b->scan_bytes + 256);
 }
 
@@ -685,6 +758,7 @@ int qp_beam_destroy(qp_beam *b)
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     for (int k = 0; k < 7; ++k) { b->a[k].release(); b->b[k].release(); }
     b->alive.release(); b->pos.release(); b->scan_tmp.release(); b->moments.release();
+    for (int k = 0; k < 2; ++k) { b->key[k].release(); b->perm[k].release(); }
     b->q3.release();
     delete b;
     return QP_OK;
@@ -767,24 +841,32 @@ int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf, double 
     const size_t vol = (size_t)(c->nz + 1) * c->rplane;
     dim3 grid((unsigned)((np + 255) / 256));
     const double qtmh = b->qbm * b->dt, dtcx = b->dt / dex, dtcz = b->dt / dez;
-    ProfScope _ps(c, QP_PROF_BEAM_PUSH, 9.0 * 8.0 * (double)np);   // 6 read, 6 written, 3 q/flags
+    const CellKey ck{c->nx, c->ny, b->key_shift, b->key_drop};
+    {
+    ProfScope _ps(c, QP_PROF_BEAM_PUSH, (12.0 * 8.0 + 12.0) * (double)np);   // 6 read, 6 written, flag, key, index
     if (c->prec == QP_FP64)
         hipLaunchKernelGGL(k_beam_push<double>, grid, dim3(256), 0, c->stream, np,
                            b->a[0].as<double>(), b->a[1].as<double>(), b->a[2].as<double>(),
                            b->a[3].as<double>(), b->a[4].as<double>(), b->a[5].as<double>(),
                            ef->buf.as<double>(), bf->buf.as<double>(), vol, c->ldr, c->rplane,
                            qtmh, dtcx, dtcz, (double)(c->nx - 1), (double)(c->ny - 1),
-                           (double)(c->nz - 1), b->alive.as<int>());
+                           (double)(c->nz - 1), b->alive.as<int>(), ck, b->key[0].as<unsigned>(),
+                           b->perm[0].as<unsigned>());
     else
         hipLaunchKernelGGL(k_beam_push<float>, grid, dim3(256), 0, c->stream, np,
                            b->a[0].as<double>(), b->a[1].as<double>(), b->a[2].as<double>(),
                            b->a[3].as<double>(), b->a[4].as<double>(), b->a[5].as<double>(),
                            ef->buf.as<float>(), bf->buf.as<float>(), vol, c->ldr, c->rplane,
                            qtmh, dtcx, dtcz, (double)(c->nx - 1), (double)(c->ny - 1),
-                           (double)(c->nz - 1), b->alive.as<int>());
+                           (double)(c->nz - 1), b->alive.as<int>(), ck, b->key[0].as<unsigned>(),
+                           b->perm[0].as<unsigned>());
+    }
     QP_LAUNCH_CHECK();
-    // order-preserving compaction of the survivors (the reference instead
-    // moves the last particle into each hole, part3d_lib77.f:748-776)
+    // The survivors, sorted by cell (a stable sort: equal cells keep their
+    // order; the dropped particles' key sorts them last). The reference
+    // instead moves the last particle into each hole (part3d_lib77.f:748-776);
+    // the survivor set is the same, and the order is the one the gathers of
+    // the next push and the atomics of the next charge deposit want.
     QP_HIP(hipMemsetAsync(b->alive.as<int>() + np, 0, sizeof(int), c->stream));
     size_t tb = b->scan_bytes;
     QP_HIP(hipcub::DeviceScan::ExclusiveSum(b->scan_tmp.p, tb, b->alive.as<int>(),
@@ -792,12 +874,26 @@ int qp_beam_push(qp_beam *b, const qp_field3d *ef, const qp_field3d *bf, double 
     int live = 0;
     QP_HIP(hipMemcpyAsync(&live, b->pos.as<int>() + np, sizeof(int), hipMemcpyDeviceToHost,
                           c->stream));
+    {
+    ProfScope _ps(c, QP_PROF_BEAM_PUSH, 16.0 * (double)np);   // keys and indices read and written once
+    size_t need = 0;
+    QP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, b->key[0].as<unsigned>(),
+                                              b->key[1].as<unsigned>(), b->perm[0].as<unsigned>(),
+                                              b->perm[1].as<unsigned>(), (int)np, 0, b->key_bits,
+                                              c->stream));
+    QP_REQUIRE(need <= b->scan_bytes, QP_ERR_ALLOC, "beam push: sort needs %zu bytes, %zu held",
+               need, b->scan_bytes);
+    QP_HIP(hipcub::DeviceRadixSort::SortPairs(b->scan_tmp.p, need, b->key[0].as<unsigned>(),
+                                              b->key[1].as<unsigned>(), b->perm[0].as<unsigned>(),
+                                              b->perm[1].as<unsigned>(), (int)np, 0, b->key_bits,
+                                              c->stream));
+    }
     Ptr7 src, dst;
     for (int k = 0; k < 7; ++k) { src.p[k] = b->a[k].as<double>(); dst.p[k] = b->b[k].as<double>(); }
     {
-    ProfScope _ps(c, QP_PROF_BEAM_PUSH, 7.0 * 8.0 * 2.0 * (double)np);
-    hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, c->stream, np, b->alive.as<int>(),
-                       b->pos.as<int>(), src, dst);
+    ProfScope _ps(c, QP_PROF_BEAM_PUSH, (7.0 * 8.0 * 2.0 + 4.0) * (double)np);
+    hipLaunchKernelGGL(k_permute, grid, dim3(256), 0, c->stream, np, b->pos.as<int>() + np,
+                       b->perm[1].as<unsigned>(), src, dst);
     }
     QP_LAUNCH_CHECK();
     QP_HIP(hipStreamSynchronize(c->stream));

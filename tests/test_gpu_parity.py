@@ -389,3 +389,41 @@ def test_beam_push(ctx):
     assert 0 < got.shape[0] < part.shape[0]
     assert got.shape == want.shape              # the same survivors
     assert rel(canon(got), canon(want)) < 1e-12
+
+
+def test_beam_push_sorts_by_cell_and_deposit_ignores_order(ctx):
+    """The push leaves the survivors sorted by cell (z, then y, then x); the
+    charge deposit, whose runs of equal cells are summed across the
+    wavefront before the atomics, gives the same 3-D charge, bit for bit,
+    for the sorted beam and for the same particles reversed or shuffled, and
+    agrees with the oracle's particle-by-particle deposit."""
+    banner("beam cell sort + deposit order")
+    grid = qpo.Grid(IND, IND, 6)
+    # a dense core: many particles per cell, so runs of equal cells are long
+    spec = qpo.BeamSpec(np_=(32, 32, 64), peak=4.0, sigma=(0.6, 0.6, 0.6),
+                        center=(0.0, 0.0, 4.0))
+    b = qps.Beam(ctx, center=spec.center, sigma=spec.sigma, np_=spec.np_, peak=spec.peak,
+                 dt=200.0)
+    rng = np.random.default_rng(29)
+    fe, fb = qps.Field3d(ctx, 3), qps.Field3d(ctx, 3)
+    fe.upload(rng.normal(0, 0.05, (3, 65, 65, 65)))
+    fb.upload(rng.normal(0, 0.05, (3, 65, 65, 65)))
+    b.push(fe, fb, grid.dx, grid.dz)
+    got = b.download()
+    assert got.shape[0] > 0
+    cells = (got[:, 2].astype(np.int64) * grid.ny + got[:, 1].astype(np.int64)) * grid.nx \
+        + got[:, 0].astype(np.int64)
+    assert np.all(np.diff(cells) >= 0)          # sorted by cell
+    assert np.unique(cells, return_counts=True)[1].max() > 64   # runs fill whole wavefronts
+    b.qdp()
+    q_sorted = b.charge()
+    want = np.zeros((1, 65, 65, 65))
+    qpo.beam_deposit(got, want)
+    assert rel(q_sorted, want[0]) < 1e-12
+    for perm in (np.arange(got.shape[0])[::-1], rng.permutation(got.shape[0])):
+        b2 = qps.Beam(ctx, np_=(0, 0, 0), npmax=got.shape[0] + 2, dt=200.0)
+        b2.upload(got[perm])
+        b2.qdp()
+        assert np.array_equal(b2.charge(), q_sorted)
+        b2.close()
+    b.close()
